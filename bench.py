@@ -1,0 +1,208 @@
+"""Benchmark of the one-GPU radix hash join (prints ONE JSON line).
+
+Workload (default): R = S = 640M tuples (uint32 key + uint32 row-index value,
+5.1 GB per relation), R keys distinct in [1, 2^31), 10% of S drawn from R, the
+rest distinct in [2^31, 2^32); two-level radix with 14 total bits and
+partition_size 512; blocked Bloom filter at 10 bits/key with k = 3 built from R
+to filter S.  Inputs are generated on the device and are resident in HBM before
+the timed region.  A step is one full join of the two relations.
+
+value = (N_R + N_S) / (wall seconds per step), timed over K steps between
+torch.cuda.synchronize() calls.  roofline = the dominant kernel's algorithmic
+bytes / its average duration (HIP events on the library's stream, over the timed
+steps) against 8 TB/s.  cpu_baseline = the reference's own CPU join
+(src/sanity_check.cpp compiled into oracle/_ref), or the oracle's restatement
+of it, on a bounded sample of the same distribution, all host threads.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "brig-join_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (N_R, N_S, match ratio, zipf_s, params)
+    "join640m_bf10": (640_000_000, 640_000_000, 0.10, 0.0,
+                      dict(partition_size=512, radix_bits=14, use_bf=1, bf_bits_per_key=10.0,
+                           num_hashes=3)),
+    "join16m_nobf": (16_000_000, 16_000_000, 0.10, 0.0, dict(partition_size=1024)),
+    "join640m_nobf": (640_000_000, 640_000_000, 0.10, 0.0, dict(partition_size=512, radix_bits=14)),
+    "join64m_1280m_zipf": (64_000_000, 1_280_000_000, 0.01, 1.0,
+                           dict(use_bf=1, bf_bits_per_key=12.0, num_hashes=4)),
+}
+
+
+def algorithmic_bytes(kernel: str, n_r: int, n_s: int, m: int, m_s: int, bf_bytes: int,
+                      bloom: bool) -> float:
+    """Compulsory streamed bytes of one launch of a kernel:
+    a partition pass reads and writes 8 B per tuple, build+probe read 8 B per R
+    and per S' tuple and write 12 B per row, the Bloom build reads 4 B per R key
+    and writes the filter once, the Bloom filter reads 8 B per S tuple and writes
+    8 B per matching S tuple; histogram passes read 4 B per key."""
+    s_part = m_s if bloom else n_s
+    return {
+        "hist_r": 4.0 * n_r, "scatter1_r": 16.0 * n_r, "scatter2_r": 16.0 * n_r,
+        "bf_build": 4.0 * n_r + bf_bytes, "bf_filter": 8.0 * n_s + 8.0 * m_s,
+        "hist_s": 4.0 * s_part, "scatter1_s": 16.0 * s_part, "scatter2_s": 16.0 * s_part,
+        "probe": 8.0 * n_r + 8.0 * s_part + 12.0 * m,
+    }.get(kernel, 0.0)
+
+
+def total_algorithmic_bytes(n_r, n_s, m, m_s, bf_bytes, bloom) -> float:
+    """Whole-join algorithmic bytes, two partition passes (BASELINE.md formulas)."""
+    if bloom:
+        return 44.0 * n_r + bf_bytes + 8.0 * n_s + 48.0 * m_s + 12.0 * m
+    return 40.0 * (n_r + n_s) + 12.0 * m
+
+
+def cpu_baseline(n_r_full: int, n_s_full: int, ratio: float, budget_s: float = 12.0) -> dict:
+    """Time the reference's CPU join on a bounded sample of the workload's distribution."""
+    import numpy as np
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    n = 32_000_000
+    rk, _ = O.gen_r(42, n)
+    sk, _ = O.gen_s(42, 42 ^ 0xDEADBEEF, n, n, O.match_count(n, ratio))
+    if O.ref_available():
+        kind = "reference"
+        run = lambda: O.ref_sanity_count(rk, sk)  # noqa: E731
+    else:
+        kind = "port"
+
+        def run():
+            t0 = time.perf_counter()
+            c = O.sanity_count(rk, sk, threads)
+            return c, time.perf_counter() - t0
+    c, sec = run()                         # warm-up
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 3 and (time.perf_counter() - t_start) < budget_s:
+        c, sec = run()
+        times.append(sec)
+    med = float(np.median(times))
+    assert c == O.match_count(n, ratio), (c, n)
+    return {"value": 2.0 * n / med, "unit": "input tuples/s", "cores": threads, "kind": kind,
+            "sample": f"{n // 1_000_000}M x {n // 1_000_000}M tuples, same generator "
+                      f"({ratio:.0%} match), count-only join, median of {len(times)} "
+                      f"after a warm-up; {med:.3f} s per join"}
+
+
+def load_traffic(workload: str, kernel: str):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get(kernel)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="join640m_bf10", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures the one-GPU join only: run it with --gpus 1 (no multi-GPU path "
+              "is built)", file=sys.stderr)
+        return 2
+
+    import torch
+    import brigjoin as bj
+
+    n_r, n_s, ratio, zipf_s, params = WORKLOADS[args.workload]
+    m_s = int(min(n_s, max(0, round(ratio * n_s))))
+    dev = torch.device("cuda:0")
+    rk = torch.empty(n_r, dtype=torch.int32, device=dev)
+    rv = torch.empty(n_r, dtype=torch.int32, device=dev)
+    sk = torch.empty(n_s, dtype=torch.int32, device=dev)
+    sv = torch.empty(n_s, dtype=torch.int32, device=dev)
+    ctx = bj.Context(0)
+    ctx.gen_build(42, n_r, rk, rv)
+    ctx.gen_probe(42, 42 ^ 0xDEADBEEF, n_r, n_s, m_s, sk, sv, zipf_s=zipf_s)
+    torch.cuda.synchronize()
+    p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=1, **params)
+    bloom = bool(params.get("use_bf"))
+
+    for _ in range(args.warmup):
+        res = ctx.join(p, rk, rv, sk, sv)
+    torch.cuda.synchronize()
+    kern_sum = {k: 0.0 for k in bj.KERNELS}
+    total_dev = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = ctx.join(p, rk, rv, sk, sv)
+        for k, v in res.kernels().items():
+            kern_sum[k] += v
+        total_dev += res.phase_ms[6]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if res.match_count != m_s:
+        print(f"wrong match count {res.match_count} != {m_s}", file=sys.stderr)
+        return 1
+
+    ms_step = wall / args.steps * 1e3
+    value = (n_r + n_s) / (wall / args.steps)
+    kern_avg = {k: v / args.steps for k, v in kern_sum.items()}
+    bf_bytes = int(res.bf_words) * 4
+    m = int(res.match_count)
+    ms_sv = int(res.s_survivors)
+    dominant = max((k for k in kern_avg if algorithmic_bytes(k, n_r, n_s, m, m_s, bf_bytes, bloom) > 0),
+                   key=lambda k: kern_avg[k])
+    alg = algorithmic_bytes(dominant, n_r, n_s, m, m_s, bf_bytes, bloom)
+    achieved = alg / (kern_avg[dominant] * 1e-3) / 1e9
+    traffic = load_traffic(args.workload, dominant)
+    per_kernel = {k: {"ms": round(v, 4),
+                      "alg_GBps": round(algorithmic_bytes(k, n_r, n_s, m, m_s, bf_bytes, bloom)
+                                        / (v * 1e-3) / 1e9, 1) if v > 0 else None}
+                  for k, v in kern_avg.items() if v > 0}
+    B = total_algorithmic_bytes(n_r, n_s, m, m_s, bf_bytes, bloom)
+
+    out = {
+        "metric": "input tuples/s at 640M⋈640M on 1 MI355X; % HBM roofline; batched vs plain",
+        "value": value,
+        "unit": "input tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (device generator: distinct R keys, exact match ratio)",
+        "config": {"workload": args.workload, "n_r": n_r, "n_s": n_s, "match_ratio": ratio,
+                   "zipf_s": zipf_s, **{k: v for k, v in params.items()},
+                   "bits1": int(res.bits1), "bits2": int(res.bits2), "tile": int(res.tile),
+                   "bf_seg_bits": int(res.bf_seg_bits), "bf_k": int(res.bf_k)},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "alg_bytes": alg, "avg_ms": round(kern_avg[dominant], 4)},
+        "whole_join": {"alg_bytes": B, "device_ms": round(total_dev / args.steps, 4),
+                       "alg_frac_of_8TBps": round(B / (total_dev / args.steps * 1e-3) / 8e12, 4),
+                       "matches": m, "s_survivors": ms_sv,
+                       "vlsplit_items": int(res.heavy_items)},
+        "kernels": per_kernel,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n_r, n_s, ratio)
+        out["cpu_baseline"]["gpu_speedup"] = round(value / out["cpu_baseline"]["value"], 1)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

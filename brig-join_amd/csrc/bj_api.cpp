@@ -1,0 +1,663 @@
+// bj_api.cpp -- host orchestration of the gfx950 radix hash join behind the C ABI.
+//
+// Counterparts in the reference (Chanho-LEE-SNU/BRIG-Join):
+//   make_plan      parameter derivation of gpu_main (src/hash_kernel.cu:174-284):
+//                  Bloom size/hash count formula (:188-196), partition_size and
+//                  second_partition_size sizing (:198-284)
+//   run_partition  partial_jointable_gpu_partition{,_r_only,_s_only}
+//                  (src/vlsplit_hash_kernel.cu:10-113, 224-297)
+//   bj_join        the device phase of gpu_main (src/hash_kernel.cu:636-1131) and
+//                  join_execute_host's local branch (src/join_executor.cu:14-110)
+// Unlike the reference, no host synchronisation happens between phases: every
+// size the device produces (|S'|, work items) is read by the next kernel from
+// device memory, and the host reads the counters once at the end.
+#include "bj_api.h"
+#include "bj_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+using namespace bj;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define BJ_HIP(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(BJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_N = 8 };
+
+// Event i is recorded when kernel slot i - 1 ends (slot order of BJ_K_*);
+// EV_K0 is recorded before the first hot-path kernel.
+enum { EV_H2D0 = 0, EV_H2D1, EV_K0, EV_N = EV_K0 + BJ_NUM_KERNELS + 1 };
+
+uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+PermKeys perm_keys(uint64_t seed)
+{
+    PermKeys K;
+    uint64_t st = seed;
+    for (int r = 0; r < 4; r++) {
+        st = splitmix64(st);
+        K.k[2 * r] = (uint32_t)st;
+        st = splitmix64(st);
+        K.k[2 * r + 1] = (uint32_t)(st >> 32) | 1u;
+    }
+    return K;
+}
+
+int ceil_log2(uint64_t n)
+{
+    int l = 0;
+    while ((1ull << l) < n) l++;
+    return l;
+}
+
+bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+
+}  // namespace
+
+struct bj_ctx {
+    int device = 0;
+    int num_cu = 256;
+    int max_lds = 64 * 1024;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[EV_N] = {};
+    // inputs copied from the host (only when inputs_on_device == 0)
+    DevBuf in_rk, in_rv, in_sk, in_sv;
+    // partitioned relations and scratch
+    DevBuf r_key, r_val, s_key, s_val, tmp_key, tmp_val, sp_key, sp_val;
+    DevBuf filter, part, totals, l1part, l1off, r_start, s_start, partials, items, counters;
+    DevBuf out_key, out_rval, out_sval, dtmp;
+    uint64_t* h_counters = nullptr;  // pinned
+};
+
+namespace {
+
+int ensure(DevBuf& b, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (b.bytes >= bytes) return BJ_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        (void)e;
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        return fail(BJ_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " +
+                                      hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return BJ_OK;
+}
+
+void release(DevBuf& b)
+{
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        (void)e;
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+template <typename T>
+T* P(DevBuf& b) { return (T*)b.p; }
+
+// ----------------------------------------------------------------- the plan
+struct Plan {
+    uint64_t nr = 0, ns = 0;
+    int bits = 1, bits1 = 1, bits2 = 0;
+    int G_r = 1, G_s = 1;
+    int tile = 8192;
+    uint32_t slice = 65536;
+    bool bf = false;
+    BloomGeom geom{0, 8, 1};
+    uint64_t bf_words = 0;
+    uint64_t cap = 1;
+    uint64_t item_cap = 1;
+};
+
+constexpr uint64_t MAX_TUPLES = 0x7FFFFFFFull;  // row ids are 31-bit in the probe's staging
+constexpr int MAX_BITS = 15;                     // joint histogram in LDS: 2^15 x 4 B
+constexpr int MAX_LEVEL_BITS = 11;               // fan-out per scatter pass
+constexpr uint64_t SEG_TARGET_BYTES = 64 * 1024; // Bloom segment built in LDS
+constexpr uint64_t SEG_MAX_BYTES = 160 * 1024;
+
+int choose_G(uint64_t n)
+{
+    uint64_t g = (n + 16383) / 16384;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(1024, g));
+}
+
+void split_bits(int total, int& b1, int& b2)
+{
+    if (total <= MAX_LEVEL_BITS) {
+        b1 = total;
+        b2 = 0;
+    } else {
+        b1 = (total + 1) / 2;
+        b2 = total - b1;
+    }
+}
+
+int make_plan(const bj_params& p, Plan& pl)
+{
+    if (p.r_size < 0 || p.s_size < 0) return fail(BJ_ERR_INVALID, "negative relation size");
+    if ((uint64_t)p.r_size > MAX_TUPLES || (uint64_t)p.s_size > MAX_TUPLES)
+        return fail(BJ_ERR_SIZE, "relation larger than 2^31-1 tuples");
+    pl.nr = (uint64_t)p.r_size;
+    pl.ns = (uint64_t)p.s_size;
+
+    pl.tile = p.tile > 0 ? p.tile : 8192;
+    if (pl.tile != 4096 && pl.tile != 8192) return fail(BJ_ERR_INVALID, "tile must be 4096 or 8192");
+
+    // use_bf modes of the reference: 0..16 except 11 and 12 (src/hash_kernel.cu:146-150)
+    if (p.use_bf < 0 || p.use_bf == 11 || p.use_bf == 12 || p.use_bf > 16)
+        return fail(BJ_ERR_INVALID, "unsupported use_bf=" + std::to_string(p.use_bf));
+    pl.bf = p.use_bf != 0;
+
+    int b1 = -1, b2 = -1;
+    if (p.partition_size >= 0) {
+        if (!is_pow2(p.partition_size) || p.partition_size < 2 || p.partition_size > 2048)
+            return fail(BJ_ERR_INVALID, "partition_size must be a power of two in [2, 2048]");
+        b1 = ceil_log2((uint64_t)p.partition_size);
+    }
+    if (p.second_partition_size >= 0) {
+        if (!is_pow2(p.second_partition_size) || p.second_partition_size > 2048)
+            return fail(BJ_ERR_INVALID, "second_partition_size must be a power of two in [1, 2048]");
+        b2 = ceil_log2((uint64_t)p.second_partition_size);
+    }
+
+    // Bloom geometry first: its segment count can raise the automatic radix bits.
+    double num_bits = 0.0;
+    int k = 0;
+    if (pl.bf) {
+        const double R = (double)std::max<uint64_t>(pl.nr, 1);
+        if (p.num_bits >= 0) {
+            num_bits = (double)p.num_bits;
+        } else if (p.bf_bits_per_key > 0) {
+            num_bits = R * p.bf_bits_per_key;
+        } else {
+            // reference formula, src/hash_kernel.cu:189-193
+            const double fp = p.true_false_ratio;
+            if (!(fp > 0.0 && fp < 1.0)) return fail(BJ_ERR_INVALID, "true_false_ratio must be in (0,1)");
+            num_bits = -R * std::log(fp) / (std::log(2.0) * std::log(2.0));
+        }
+        if (p.num_hashes >= 0) {
+            k = p.num_hashes;
+        } else if (p.num_bits < 0 && p.bf_bits_per_key > 0) {
+            k = (int)std::lround(p.bf_bits_per_key * std::log(2.0));
+        } else {
+            k = (int)(num_bits / R * std::log(2.0));
+        }
+        if (k < 1) k = 1;
+        if (p.num_hashes > 8) return fail(BJ_ERR_INVALID, "num_hashes must be in [1, 8]");
+        if (k > 8) k = 8;
+        if (num_bits < 256.0) num_bits = 256.0;
+    }
+    const uint64_t bf_bytes = pl.bf ? (uint64_t)std::ceil(num_bits / 8.0) : 0;
+    const int seg_need = pl.bf ? std::max(0, ceil_log2((bf_bytes + SEG_TARGET_BYTES - 1) / SEG_TARGET_BYTES)) : 0;
+
+    // radix bits
+    int total;
+    if (p.radix_bits > 0) {
+        total = p.radix_bits;
+        if (b1 >= 0 && b2 >= 0 && b1 + b2 != total)
+            return fail(BJ_ERR_INVALID, "radix_bits != log2(partition_size * second_partition_size)");
+        if (b1 >= 0 && b2 < 0) b2 = total - b1;
+        else if (b2 >= 0 && b1 < 0) b1 = total - b2;
+        else if (b1 < 0 && b2 < 0) split_bits(total, b1, b2);
+        if (b1 < 1 || b2 < 0) return fail(BJ_ERR_INVALID, "radix_bits smaller than the given fan-out");
+    } else {
+        const uint64_t m = std::max<uint64_t>(1, std::min(pl.nr, pl.ns));
+        const double per = 0.6 * pl.tile;
+        int need = std::max(1, ceil_log2((uint64_t)std::ceil((double)m / per)));
+        need = std::max(need, seg_need);
+        need = std::min(need, MAX_BITS);
+        if (b1 >= 0 && b2 >= 0) {
+        } else if (b1 >= 0) {
+            b2 = std::max(0, need - b1);
+        } else if (b2 >= 0) {
+            b1 = std::max(1, need - b2);
+        } else {
+            split_bits(need, b1, b2);
+        }
+        total = b1 + b2;
+    }
+    if (total < 1 || total > MAX_BITS)
+        return fail(BJ_ERR_SIZE, "total radix bits must be in [1, 15] (got " + std::to_string(total) + ")");
+    if (b1 > MAX_LEVEL_BITS || b2 > MAX_LEVEL_BITS)
+        return fail(BJ_ERR_SIZE, "a partition level exceeds 2048-way fan-out");
+    pl.bits = total;
+    pl.bits1 = b1;
+    pl.bits2 = b2;
+
+    if (pl.bf) {
+        int seg_bits = std::min(seg_need, total);
+        uint64_t words = (bf_bytes + 3) / 4;
+        uint64_t wps = (words + (1ull << seg_bits) - 1) >> seg_bits;
+        wps = std::max<uint64_t>(8, (wps + 7) / 8 * 8);
+        if (wps * 4 > SEG_MAX_BYTES)
+            return fail(BJ_ERR_SIZE, "Bloom segment exceeds LDS: raise radix bits or lower filter bits");
+        pl.geom.seg_bits = seg_bits;
+        pl.geom.wps = (uint32_t)wps;
+        pl.geom.k = k;
+        pl.bf_words = wps << seg_bits;
+    }
+
+    pl.G_r = choose_G(pl.nr);
+    pl.G_s = choose_G(pl.ns);
+    const double scale = std::max(1.0, p.answer_scale);
+    pl.cap = std::max<uint64_t>(1, (uint64_t)((double)std::max(pl.nr, pl.ns) * scale));
+    pl.item_cap = (1ull << total) + (pl.nr + pl.ns) / pl.tile + (pl.nr + pl.ns) / pl.slice + 1024;
+    return BJ_OK;
+}
+
+// Two-level partition of one relation into out_key/out_val with sub-bucket
+// starts in start[0 .. 2^bits].
+int run_partition(bj_ctx* c, const Plan& pl, const uint32_t* key, const uint32_t* val,
+                  uint64_t n_host, const uint64_t* n_dev, int G, uint32_t* out_key,
+                  uint32_t* out_val, uint64_t out_cap, uint64_t* start, int k_hist)
+{
+    // k_hist is the BJ_K_HIST_* slot; the three slots after it follow in order
+    auto mark = [&](int slot) { return hipEventRecord(c->ev[EV_K0 + 1 + slot], c->stream); };
+    hipStream_t st = c->stream;
+    const int nbins = 1 << pl.bits;
+    const uint64_t n_l1 = (uint64_t)(1 << pl.bits1) * G;
+    uint64_t* err = P<uint64_t>(c->counters) + CTR_ERR;
+    BJ_HIP(launch_hist(key, n_host, n_dev, G, pl.bits, P<uint32_t>(c->part), st));
+    BJ_HIP(mark(k_hist));
+    BJ_HIP(launch_reduce(P<uint32_t>(c->part), G, pl.bits, pl.bits2, P<uint32_t>(c->totals),
+                         P<uint32_t>(c->l1part), st));
+    BJ_HIP(launch_scan(P<uint32_t>(c->totals), (uint64_t)nbins, start, P<uint64_t>(c->partials), st));
+    BJ_HIP(launch_scan(P<uint32_t>(c->l1part), n_l1, P<uint64_t>(c->l1off), P<uint64_t>(c->partials), st));
+    BJ_HIP(mark(k_hist + 1));
+    if (pl.bits2 > 0) {
+        BJ_HIP(launch_scatter(key, val, P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val),
+                              c->tmp_key.bytes / 4, n_host, n_dev, G, 0, pl.bits, pl.bits2,
+                              P<uint64_t>(c->l1off), start, err, st));
+        BJ_HIP(mark(k_hist + 2));
+        BJ_HIP(launch_scatter(P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val), out_key, out_val, out_cap,
+                              n_host, n_dev, G, 1, pl.bits, pl.bits2, P<uint64_t>(c->l1off), start, err,
+                              st));
+        BJ_HIP(mark(k_hist + 3));
+    } else {
+        BJ_HIP(launch_scatter(key, val, out_key, out_val, out_cap, n_host, n_dev, G, 0, pl.bits, 0,
+                              P<uint64_t>(c->l1off), start, err, st));
+        BJ_HIP(mark(k_hist + 2));
+        BJ_HIP(mark(k_hist + 3));
+    }
+    return BJ_OK;
+}
+
+// Allocates (or, with c == nullptr, only sums) the device workspace of a join.
+int ensure_workspace(bj_ctx* c, const Plan& pl, uint64_t* total_bytes = nullptr)
+{
+    uint64_t total = 0;
+    auto need = [&](DevBuf* b, uint64_t bytes) -> int {
+        bytes = std::max<uint64_t>(bytes, 256);
+        total += bytes;
+        return c ? ensure(*b, bytes) : BJ_OK;
+    };
+    auto B = [&](DevBuf bj_ctx::*m) -> DevBuf* { return c ? &(c->*m) : nullptr; };
+    int rc;
+    const uint64_t nr = std::max<uint64_t>(pl.nr, 1), ns = std::max<uint64_t>(pl.ns, 1);
+    const uint64_t nbins = 1ull << pl.bits;
+    const uint64_t gmax = (uint64_t)std::max(pl.G_r, pl.G_s);
+    const uint64_t n_l1 = (1ull << pl.bits1) * gmax;
+    if ((rc = need(B(&bj_ctx::r_key), nr * 4)) || (rc = need(B(&bj_ctx::r_val), nr * 4))) return rc;
+    if ((rc = need(B(&bj_ctx::s_key), ns * 4)) || (rc = need(B(&bj_ctx::s_val), ns * 4))) return rc;
+    if (pl.bits2 > 0) {
+        const uint64_t t = std::max(nr, ns);
+        if ((rc = need(B(&bj_ctx::tmp_key), t * 4)) || (rc = need(B(&bj_ctx::tmp_val), t * 4))) return rc;
+    }
+    if (pl.bf) {
+        if ((rc = need(B(&bj_ctx::sp_key), ns * 4)) || (rc = need(B(&bj_ctx::sp_val), ns * 4))) return rc;
+        if ((rc = need(B(&bj_ctx::filter), pl.bf_words * 4))) return rc;
+    }
+    if ((rc = need(B(&bj_ctx::part), gmax * nbins * 4))) return rc;
+    if ((rc = need(B(&bj_ctx::totals), nbins * 4))) return rc;
+    if ((rc = need(B(&bj_ctx::l1part), n_l1 * 4))) return rc;
+    if ((rc = need(B(&bj_ctx::l1off), (n_l1 + 1) * 8))) return rc;
+    if ((rc = need(B(&bj_ctx::r_start), (nbins + 1) * 8)) || (rc = need(B(&bj_ctx::s_start), (nbins + 1) * 8)))
+        return rc;
+    if ((rc = need(B(&bj_ctx::partials), scan_partials_needed(std::max(nbins, n_l1)) * 8 + 64))) return rc;
+    if ((rc = need(B(&bj_ctx::items), pl.item_cap * sizeof(HeavyItem)))) return rc;
+    if ((rc = need(B(&bj_ctx::out_key), pl.cap * 4)) || (rc = need(B(&bj_ctx::out_rval), pl.cap * 4)) ||
+        (rc = need(B(&bj_ctx::out_sval), pl.cap * 4)))
+        return rc;
+    if (total_bytes) *total_bytes = total;
+    return BJ_OK;
+}
+
+float ev_ms(bj_ctx* c, int a, int b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int bj_version(void) { return BJ_API_VERSION; }
+
+const char* bj_strerror(int code)
+{
+    switch (code) {
+    case BJ_OK: return "ok";
+    case BJ_ERR_INVALID: return "invalid argument";
+    case BJ_ERR_HIP: return "HIP runtime error";
+    case BJ_ERR_NOMEM: return "out of memory";
+    case BJ_ERR_IO: return "I/O error";
+    case BJ_ERR_SIZE: return "size exceeds a supported limit";
+    case BJ_ERR_NODEVICE: return "no such HIP device";
+    case BJ_ERR_INTERNAL: return "device-side consistency check failed";
+    default: return "unknown error";
+    }
+}
+
+const char* bj_last_error(void) { return g_last_error.c_str(); }
+
+void bj_params_default(bj_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->partition_size = -1;
+    p->second_partition_size = -1;
+    p->radix_bits = -1;
+    p->use_bf = 0;
+    p->bf_bits_per_key = 0.0;
+    p->true_false_ratio = 0.1;
+    p->num_bits = -1;
+    p->num_hashes = -1;
+    p->answer_scale = 1.0;
+    p->tile = 0;
+    p->inputs_on_device = 0;
+    p->collect_phase_times = 1;
+    p->s_chunks = 1;
+}
+
+int bj_create(bj_ctx** out, int device)
+{
+    if (!out) return fail(BJ_ERR_INVALID, "null ctx pointer");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return fail(BJ_ERR_NODEVICE, "no HIP device");
+    }
+    if (device < 0 || device >= ndev) return fail(BJ_ERR_NODEVICE, "device index out of range");
+    BJ_HIP(hipSetDevice(device));
+    bj_ctx* c = new bj_ctx();
+    c->device = device;
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
+        c->num_cu = cu;
+    int max_lds = 64 * 1024;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        max_lds = 64 * 1024;
+    c->max_lds = max_lds;
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = kernels_init(max_lds);
+    for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_counters, CTR_N * sizeof(uint64_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        bj_destroy(c);
+        return fail(BJ_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    c->stream = c->own_stream;
+    int rc = ensure(c->counters, CTR_N * sizeof(uint64_t));
+    if (rc == BJ_OK) rc = ensure(c->dtmp, 64);
+    if (rc != BJ_OK) {
+        bj_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return BJ_OK;
+}
+
+int bj_destroy(bj_ctx* c)
+{
+    if (!c) return BJ_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->in_rk,  &c->in_rv,    &c->in_sk,   &c->in_sv,    &c->r_key,   &c->r_val,
+                      &c->s_key,  &c->s_val,    &c->tmp_key, &c->tmp_val,  &c->sp_key,  &c->sp_val,
+                      &c->filter, &c->part,     &c->totals,  &c->l1part,   &c->l1off,   &c->r_start,
+                      &c->s_start, &c->partials, &c->items,  &c->counters, &c->out_key, &c->out_rval,
+                      &c->out_sval, &c->dtmp};
+    for (DevBuf* b : bufs) release(*b);
+    for (int i = 0; i < EV_N; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return BJ_OK;
+}
+
+int bj_set_stream(bj_ctx* c, void* s)
+{
+    if (!c) return fail(BJ_ERR_INVALID, "null ctx");
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return BJ_OK;
+}
+
+int bj_plan(const bj_params* p, bj_plan_info* out)
+{
+    if (!p || !out) return fail(BJ_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    Plan pl;
+    int rc = make_plan(*p, pl);
+    if (rc) return rc;
+    uint64_t ws = 0;
+    ensure_workspace(nullptr, pl, &ws);
+    if (!p->inputs_on_device) ws += (pl.nr + pl.ns) * 8;
+    out->bits1 = pl.bits1;
+    out->bits2 = pl.bits2;
+    out->bf = pl.bf ? 1 : 0;
+    out->bf_seg_bits = pl.bf ? pl.geom.seg_bits : 0;
+    out->bf_k = pl.bf ? pl.geom.k : 0;
+    out->bf_words = pl.bf_words;
+    out->tile = pl.tile;
+    out->capacity = pl.cap;
+    out->workspace_bytes = ws;
+    return BJ_OK;
+}
+
+int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t* r_val,
+            const uint32_t* s_key, const uint32_t* s_val, bj_result* out)
+{
+    if (!c || !p || !out) return fail(BJ_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    Plan pl;
+    int rc = make_plan(*p, pl);
+    if (rc) return rc;
+    if ((pl.nr && (!r_key || !r_val)) || (pl.ns && (!s_key || !s_val)))
+        return fail(BJ_ERR_INVALID, "null relation pointer");
+    BJ_HIP(hipSetDevice(c->device));
+    if ((rc = ensure_workspace(c, pl))) return rc;
+    hipStream_t st = c->stream;
+
+    // host inputs: stage them on the device first (timed as BJ_PH_H2D)
+    BJ_HIP(hipEventRecord(c->ev[EV_H2D0], st));
+    if (!p->inputs_on_device) {
+        if ((rc = ensure(c->in_rk, pl.nr * 4)) || (rc = ensure(c->in_rv, pl.nr * 4)) ||
+            (rc = ensure(c->in_sk, pl.ns * 4)) || (rc = ensure(c->in_sv, pl.ns * 4)))
+            return rc;
+        if (pl.nr) {
+            BJ_HIP(hipMemcpyAsync(c->in_rk.p, r_key, pl.nr * 4, hipMemcpyHostToDevice, st));
+            BJ_HIP(hipMemcpyAsync(c->in_rv.p, r_val, pl.nr * 4, hipMemcpyHostToDevice, st));
+        }
+        if (pl.ns) {
+            BJ_HIP(hipMemcpyAsync(c->in_sk.p, s_key, pl.ns * 4, hipMemcpyHostToDevice, st));
+            BJ_HIP(hipMemcpyAsync(c->in_sv.p, s_val, pl.ns * 4, hipMemcpyHostToDevice, st));
+        }
+        r_key = P<uint32_t>(c->in_rk);
+        r_val = P<uint32_t>(c->in_rv);
+        s_key = P<uint32_t>(c->in_sk);
+        s_val = P<uint32_t>(c->in_sv);
+    }
+    BJ_HIP(hipEventRecord(c->ev[EV_H2D1], st));
+
+    uint64_t* ctr = P<uint64_t>(c->counters);
+    BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
+    auto mark = [&](int slot) { return hipEventRecord(c->ev[EV_K0 + 1 + slot], st); };
+    BJ_HIP(hipEventRecord(c->ev[EV_K0], st));
+
+    // R: two-level partition
+    if ((rc = run_partition(c, pl, r_key, r_val, pl.nr, nullptr, pl.G_r, P<uint32_t>(c->r_key),
+                            P<uint32_t>(c->r_val), c->r_key.bytes / 4, P<uint64_t>(c->r_start),
+                            BJ_K_HIST_R)))
+        return rc;
+
+    if (pl.bf) {
+        BJ_HIP(launch_bloom_build(P<uint32_t>(c->r_key), P<uint64_t>(c->r_start), pl.bits, pl.geom,
+                                  P<uint32_t>(c->filter), st));
+        BJ_HIP(mark(BJ_K_BF_BUILD));
+        BJ_HIP(launch_bloom_filter(s_key, s_val, pl.ns, P<uint32_t>(c->filter), pl.geom,
+                                   P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), c->sp_key.bytes / 4,
+                                   ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
+        BJ_HIP(mark(BJ_K_BF_FILTER));
+        if ((rc = run_partition(c, pl, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns,
+                                ctr + CTR_SPRIME, pl.G_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val),
+                                c->s_key.bytes / 4, P<uint64_t>(c->s_start), BJ_K_HIST_S)))
+            return rc;
+    } else {
+        BJ_HIP(mark(BJ_K_BF_BUILD));
+        BJ_HIP(mark(BJ_K_BF_FILTER));
+        if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->s_key),
+                                P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint64_t>(c->s_start),
+                                BJ_K_HIST_S)))
+            return rc;
+    }
+
+    BJ_HIP(launch_plan_probe(P<uint64_t>(c->r_start), P<uint64_t>(c->s_start), pl.bits, (uint32_t)pl.tile,
+                             pl.slice, P<HeavyItem>(c->items), pl.item_cap, ctr + CTR_ITEMS,
+                             ctr + CTR_ERR, st));
+    BJ_HIP(mark(BJ_K_PLAN));
+    BJ_HIP(launch_probe(pl.tile, P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start),
+                        P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), P<uint64_t>(c->s_start), pl.bits,
+                        pl.slice, P<HeavyItem>(c->items), ctr + CTR_ITEMS, ctr + CTR_WORK, ctr + CTR_OUT,
+                        pl.cap, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
+                        P<uint32_t>(c->out_sval), c->num_cu, st));
+    BJ_HIP(mark(BJ_K_PROBE));
+
+    BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    BJ_HIP(hipStreamSynchronize(st));
+
+    const uint64_t* h = c->h_counters;
+    if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
+    out->match_count = h[CTR_OUT];
+    out->capacity = pl.cap;
+    out->rows_written = std::min(h[CTR_OUT], pl.cap);
+    out->truncated = h[CTR_OUT] > pl.cap ? 1 : 0;
+    out->s_survivors = pl.bf ? h[CTR_SPRIME] : pl.ns;
+    out->key = P<uint32_t>(c->out_key);
+    out->r_val = P<uint32_t>(c->out_rval);
+    out->s_val = P<uint32_t>(c->out_sval);
+    out->bits1 = pl.bits1;
+    out->bits2 = pl.bits2;
+    out->bf_seg_bits = pl.bf ? pl.geom.seg_bits : 0;
+    out->bf_k = pl.bf ? pl.geom.k : 0;
+    out->bf_words = pl.bf_words;
+    out->tile = pl.tile;
+    out->heavy_items = h[CTR_ITEMS];
+    if (p->collect_phase_times) {
+        for (int k = 0; k < BJ_NUM_KERNELS; k++) out->kernel_ms[k] = ev_ms(c, EV_K0 + k, EV_K0 + k + 1);
+        auto span = [&](int k_first, int k_last) { return ev_ms(c, EV_K0 + k_first, EV_K0 + k_last + 1); };
+        out->phase_ms[BJ_PH_H2D] = ev_ms(c, EV_H2D0, EV_H2D1);
+        out->phase_ms[BJ_PH_R_PART] = span(BJ_K_HIST_R, BJ_K_SCATTER2_R);
+        out->phase_ms[BJ_PH_BF_BUILD] = span(BJ_K_BF_BUILD, BJ_K_BF_BUILD);
+        out->phase_ms[BJ_PH_S_FILTER] = span(BJ_K_BF_FILTER, BJ_K_BF_FILTER);
+        out->phase_ms[BJ_PH_S_PART] = span(BJ_K_HIST_S, BJ_K_SCATTER2_S);
+        out->phase_ms[BJ_PH_PROBE] = span(BJ_K_PLAN, BJ_K_PROBE);
+        out->phase_ms[BJ_PH_TOTAL] = span(BJ_K_HIST_R, BJ_K_PROBE);
+    }
+    return BJ_OK;
+}
+
+int bj_copy_rows_to_host(bj_ctx* c, const bj_result* r, uint32_t* key, uint32_t* r_val, uint32_t* s_val,
+                         uint64_t n)
+{
+    if (!c || !r) return fail(BJ_ERR_INVALID, "null argument");
+    if (n > r->rows_written) return fail(BJ_ERR_INVALID, "n exceeds rows_written");
+    if (n == 0) return BJ_OK;
+    BJ_HIP(hipSetDevice(c->device));
+    if (key) BJ_HIP(hipMemcpyAsync(key, r->key, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (r_val) BJ_HIP(hipMemcpyAsync(r_val, r->r_val, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (s_val) BJ_HIP(hipMemcpyAsync(s_val, r->s_val, n * 4, hipMemcpyDeviceToHost, c->stream));
+    BJ_HIP(hipStreamSynchronize(c->stream));
+    return BJ_OK;
+}
+
+int bj_gen_build(bj_ctx* c, uint64_t seed, uint64_t n_r, uint32_t* d_key, uint32_t* d_val)
+{
+    if (!c || (n_r && !d_key)) return fail(BJ_ERR_INVALID, "null argument");
+    if (n_r > MAX_TUPLES) return fail(BJ_ERR_SIZE, "n_r > 2^31-1");
+    BJ_HIP(hipSetDevice(c->device));
+    BJ_HIP(launch_gen_r(d_key, d_val, n_r, perm_keys(seed), c->stream));
+    BJ_HIP(hipStreamSynchronize(c->stream));
+    return BJ_OK;
+}
+
+int bj_gen_probe(bj_ctx* c, uint64_t seed_r, uint64_t seed_s, uint64_t n_r, uint64_t n_s, uint64_t m_s,
+                 double zipf_s, uint32_t* d_key, uint32_t* d_val)
+{
+    if (!c || (n_s && !d_key)) return fail(BJ_ERR_INVALID, "null argument");
+    if (n_s > (1ull << 31) || n_r > MAX_TUPLES) return fail(BJ_ERR_SIZE, "size limit");
+    if (m_s > n_s) return fail(BJ_ERR_INVALID, "m_s > n_s");
+    if (m_s > 0 && n_r == 0) return fail(BJ_ERR_INVALID, "matches requested with an empty R");
+    if (zipf_s < 0.0) return fail(BJ_ERR_INVALID, "zipf_s < 0");
+    BJ_HIP(hipSetDevice(c->device));
+    int L = std::max(1, ceil_log2(n_s));
+    const uint32_t useed = (uint32_t)splitmix64(seed_s ^ 0x3333333333333333ULL);
+    const uint32_t zseed = (uint32_t)splitmix64(seed_s ^ 0x4444444444444444ULL);
+    BJ_HIP(launch_gen_s(d_key, d_val, n_s, n_r, m_s, L, perm_keys(seed_r),
+                        perm_keys(seed_s ^ 0x1111111111111111ULL),
+                        perm_keys(seed_s ^ 0x2222222222222222ULL), useed, zipf_s, zseed,
+                        P<double>(c->dtmp), c->stream));
+    BJ_HIP(hipStreamSynchronize(c->stream));
+    return BJ_OK;
+}
+
+}  // extern "C"

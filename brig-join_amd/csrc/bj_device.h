@@ -1,0 +1,163 @@
+// bj_device.h -- device helpers for the gfx950 radix hash join.
+//
+// Integer-only hashing, the synthetic generator's permutation, blocked Bloom
+// addressing and wave64 scan primitives.  Everything here is exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bj {
+
+constexpr int WAVE = 64;
+
+// lowbias32 (C. Wellons' published integer hash).  A bijection on uint32, so
+// distinct keys keep distinct hashes and equal keys always share a bucket.
+__host__ __device__ __forceinline__ uint32_t lowbias32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+// Radix hash: partition bucket = top `bits` bits; the LDS hash-table bucket of
+// the build/probe kernel takes the bits just below them.
+__device__ __forceinline__ uint32_t radix_hash(uint32_t key) { return lowbias32(key); }
+
+// ---------------------------------------------------------------- generator
+// 4 rounds of (add, odd multiply, xor-shift right) mod 2^w: each step is a
+// bijection on [0, 2^w), so the composition is a keyed permutation.
+struct PermKeys { uint32_t k[8]; };
+
+__host__ __device__ __forceinline__ uint32_t permw(uint32_t x, int w, const PermKeys& K)
+{
+    const uint32_t mask = (w >= 32) ? 0xFFFFFFFFu : ((1u << w) - 1u);
+    const int sh = (w + 1) / 2;
+    x &= mask;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        x = (x + K.k[2 * r]) & mask;
+        x = (x * K.k[2 * r + 1]) & mask;
+        x ^= x >> sh;
+    }
+    return x;
+}
+
+// R key of row i: the 31-bit permutation cycle-walked onto [0, 2^31 - 1), +1.
+// Distinct for distinct i and always in [1, 2^31).
+__host__ __device__ __forceinline__ uint32_t gen_r_key(uint32_t i, const PermKeys& K)
+{
+    uint32_t y = permw(i, 31, K);
+    while (y == 0x7FFFFFFFu) y = permw(y, 31, K);
+    return y + 1u;
+}
+
+// ---------------------------------------------------------------- Bloom
+// Blocked filter: 2^seg_bits segments of `wps` words (a multiple of 8); a key's
+// k bits all live in one 256-bit block of its segment.  The segment is the top
+// seg_bits of the radix hash, so segment s holds exactly the R keys of the
+// sub-buckets [s << (bits-seg_bits), (s+1) << (bits-seg_bits)).
+struct BloomGeom {
+    int seg_bits;
+    uint32_t wps;   // words per segment
+    int k;          // bits per key, 1..8
+};
+
+struct BloomAddr {
+    uint64_t block_word;  // first word of the 256-bit block
+    uint32_t mask[8];     // per-word bit mask inside the block
+};
+
+__device__ __forceinline__ void bloom_addr(uint32_t key, const BloomGeom& g, BloomAddr& a)
+{
+    const uint32_t h = radix_hash(key);
+    const uint64_t seg = g.seg_bits ? (uint64_t)(h >> (32 - g.seg_bits)) : 0ull;
+    const uint32_t rest = g.seg_bits ? (h << g.seg_bits) : h;
+    const uint64_t nblk = g.wps >> 3;
+    const uint64_t blk = ((uint64_t)rest * nblk) >> 32;
+    a.block_word = seg * g.wps + blk * 8ull;
+    const uint32_t h2 = lowbias32(key ^ 0x9E3779B9u);
+    const uint32_t h3 = lowbias32(h2 ^ 0x85EBCA6Bu);
+#pragma unroll
+    for (int w = 0; w < 8; w++) a.mask[w] = 0u;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        if (i < g.k) {
+            const uint32_t bit = (i < 4) ? ((h2 >> (8 * i)) & 255u) : ((h3 >> (8 * (i - 4))) & 255u);
+            // select the word without dynamic register indexing
+#pragma unroll
+            for (int w = 0; w < 8; w++)
+                a.mask[w] |= ((bit >> 5) == (uint32_t)w) ? (1u << (bit & 31u)) : 0u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- wave64 scans
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// number of set bits of `m` below this lane
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T x)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        T y = __shfl_up(x, o, WAVE);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+// Block-wide exclusive scan of one value per thread.  `wtmp` holds
+// BLOCK/64 + 1 entries of LDS.  Returns the exclusive prefix; *total = sum.
+template <int BLOCK, typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* wtmp, T* total)
+{
+    constexpr int NW = BLOCK / WAVE;
+    const int wid = threadIdx.x / WAVE;
+    const uint32_t lane = lane_id();
+    const T inc = wave_inclusive_scan<T>(v);
+    if (lane == WAVE - 1) wtmp[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        T w = (lane < (uint32_t)NW) ? wtmp[lane] : (T)0;
+        T wi = wave_inclusive_scan<T>(w);
+        if (lane < (uint32_t)NW) wtmp[lane] = wi - w;
+        if (lane == (uint32_t)(NW - 1)) wtmp[NW] = wi;
+    }
+    __syncthreads();
+    const T res = wtmp[wid] + inc - v;
+    *total = wtmp[NW];
+    __syncthreads();
+    return res;
+}
+
+// Exclusive scan of an LDS array in[0..n) into out[0..n) (may alias).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t lds_exclusive_scan(const uint32_t* in, uint32_t* out, int n,
+                                                      uint32_t* wtmp)
+{
+    const int per = (n + BLOCK - 1) / BLOCK;
+    const int lo = min((int)threadIdx.x * per, n);
+    const int hi = min(lo + per, n);
+    uint32_t s = 0;
+    for (int i = lo; i < hi; i++) s += in[i];
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan<BLOCK, uint32_t>(s, wtmp, &tot);
+    for (int i = lo; i < hi; i++) {
+        const uint32_t c = in[i];
+        out[i] = ex;
+        ex += c;
+    }
+    return tot;
+}
+
+}  // namespace bj

@@ -1,0 +1,55 @@
+// bj_kernels.h -- launchers of the gfx950 kernels (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bj_device.h"
+
+namespace bj {
+
+// One explicit vlsplit work item of an oversized sub-bucket.
+struct HeavyItem {
+    uint32_t sb;       // sub-bucket
+    uint32_t chunk;    // build-side chunk (tile tuples each)
+    uint32_t slice;    // probe-side slice index
+    uint32_t nslices;  // number of probe-side slices of this sub-bucket
+};
+
+hipError_t kernels_init(int max_lds_bytes);
+
+hipError_t launch_zero_u64(uint64_t* p, int n, hipStream_t st);
+hipError_t launch_gen_r(uint32_t* key, uint32_t* val, uint64_t n, const PermKeys& K, hipStream_t st);
+hipError_t launch_gen_s(uint32_t* key, uint32_t* val, uint64_t n_s, uint64_t n_r, uint64_t m_s,
+                        int L, const PermKeys& KR, const PermKeys& KP, const PermKeys& KN,
+                        uint32_t useed, double zipf_s, uint32_t zseed, double* d_tmp,
+                        hipStream_t st);
+
+hipError_t launch_hist(const uint32_t* key, uint64_t n_host, const uint64_t* n_dev, int G, int bits,
+                       uint32_t* part, hipStream_t st);
+hipError_t launch_reduce(const uint32_t* part, int G, int bits, int bits2, uint32_t* totals,
+                         uint32_t* l1part, hipStream_t st);
+size_t scan_partials_needed(uint64_t n);
+hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* partial,
+                       hipStream_t st);
+size_t scatter_lds_bytes(int nbk);
+hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* okey, uint32_t* oval,
+                          uint64_t out_cap, uint64_t n_host, const uint64_t* n_dev, int G, int mode,
+                          int bits, int bits2, const uint64_t* l1off, const uint64_t* sb_start,
+                          uint64_t* err, hipStream_t st);
+hipError_t launch_bloom_build(const uint32_t* rkey, const uint64_t* r_start, int bits,
+                              const BloomGeom& g, uint32_t* filter, hipStream_t st);
+hipError_t launch_bloom_filter(const uint32_t* skey, const uint32_t* sval, uint64_t n,
+                               const uint32_t* filter, const BloomGeom& g, uint32_t* okey,
+                               uint32_t* oval, uint64_t out_cap, uint64_t* count, uint64_t* err,
+                               int num_cu, hipStream_t st);
+hipError_t launch_plan_probe(const uint64_t* r_start, const uint64_t* s_start, int bits,
+                             uint32_t tile, uint32_t slice, HeavyItem* items, uint64_t item_cap,
+                             uint64_t* n_items, uint64_t* err, hipStream_t st);
+hipError_t launch_probe(int tile, const uint32_t* rk, const uint32_t* rv, const uint64_t* r_start,
+                        const uint32_t* sk, const uint32_t* sv, const uint64_t* s_start, int bits,
+                        uint32_t slice, const HeavyItem* items, const uint64_t* n_items,
+                        uint64_t* work_ctr, uint64_t* out_count, uint64_t out_cap, uint32_t* o_key,
+                        uint32_t* o_rval, uint32_t* o_sval, int num_cu, hipStream_t st);
+
+}  // namespace bj

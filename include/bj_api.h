@@ -1,0 +1,251 @@
+/*
+ * bj_api.h -- C ABI of the MI355X (gfx950) radix hash join.
+ *
+ * Plain C types only (no HIP, no torch, no C++ in the signatures).  Every entry
+ * point returns BJ_OK (0) or a negative BJ_ERR_* code and never exits the
+ * process.  A context is single-threaded; independent contexts may be used from
+ * different host threads.  The library owns all device memory of a context; the
+ * caller owns its inputs.
+ *
+ * What each entry point replaces in the reference (Chanho-LEE-SNU/BRIG-Join):
+ *   bj_gpu_main      gpu_main(...)                 src/hash_kernel.h:21-34,
+ *                                                  src/hash_kernel.cu:103-1654
+ *                    (its one-GPU share: read inputs, derive parameters, join,
+ *                    count; plus the output.bin that the reference parses but
+ *                    never writes, src/driver.cpp:50,119)
+ *   bj_join          the device phase of gpu_main plus join_execute_host(ctx,
+ *                    s, s_val, n) (src/join_executor.h:87-88, local branch of
+ *                    src/join_executor.cu:14-110, 351-353) and the partition
+ *                    drivers partial_jointable_gpu_partition*(...)
+ *                    (src/vlsplit_hash_kernel.h:15-60)
+ *   bj_cpu_sanity_count  run_sanity_check(...)     src/sanity_check.h:5-6
+ *   bj_gen_build / bj_gen_probe   the dataset generator gen_dataset main()
+ *                    (datagen/gen_dataset.cpp:79-263), with the synthetic
+ *                    distribution of the benchmark workloads, on the device
+ *   bj_write_output / bj_read_output  output.bin (format defined below; the
+ *                    reference defines none)
+ */
+#ifndef BJ_API_H
+#define BJ_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BJ_API_VERSION 1
+
+/* ---- error codes ------------------------------------------------------- */
+#define BJ_OK              0
+#define BJ_ERR_INVALID    -1  /* invalid argument or unsupported mode         */
+#define BJ_ERR_HIP        -2  /* HIP runtime error                            */
+#define BJ_ERR_NOMEM      -3  /* device or host allocation failed             */
+#define BJ_ERR_IO         -4  /* file open / read / write failed or truncated */
+#define BJ_ERR_SIZE       -5  /* a size exceeds a supported limit             */
+#define BJ_ERR_NODEVICE   -6  /* no HIP device / device index out of range    */
+#define BJ_ERR_INTERNAL   -7  /* a device-side consistency check failed       */
+
+/* ---- per-phase timing slots (milliseconds, hipEvent on the join stream) - */
+enum {
+    BJ_PH_H2D = 0,       /* host->device copy of host inputs (0 for device inputs) */
+    BJ_PH_R_PART,        /* two-level radix partition of R                          */
+    BJ_PH_BF_BUILD,      /* blocked Bloom filter build from partitioned R           */
+    BJ_PH_S_FILTER,      /* Bloom query + compaction S -> S'                        */
+    BJ_PH_S_PART,        /* two-level radix partition of S (or S')                  */
+    BJ_PH_PROBE,         /* LDS build + probe + row materialisation                 */
+    BJ_PH_TOTAL,         /* first hot-path kernel to last (excludes H2D)            */
+    BJ_NUM_PHASES
+};
+
+/* ---- per-kernel timing slots (milliseconds, hipEvent on the join stream) */
+enum {
+    BJ_K_HIST_R = 0,     /* k_hist over R (joint radix histogram)                 */
+    BJ_K_SCAN_R,         /* histogram reduction + offset scans of R               */
+    BJ_K_SCATTER1_R,     /* first-level scatter of R                              */
+    BJ_K_SCATTER2_R,     /* second-level scatter of R                             */
+    BJ_K_BF_BUILD,       /* Bloom build                                           */
+    BJ_K_BF_FILTER,      /* Bloom query + compaction of S                         */
+    BJ_K_HIST_S,
+    BJ_K_SCAN_S,
+    BJ_K_SCATTER1_S,
+    BJ_K_SCATTER2_S,
+    BJ_K_PLAN,           /* vlsplit work planning                                 */
+    BJ_K_PROBE,          /* LDS build + probe + rows                              */
+    BJ_NUM_KERNELS
+};
+
+/* ---- join parameters: the reference's argument surface ---------------- */
+typedef struct bj_params {
+    int64_t r_size;                 /* argv[4]  R (tuples)                            */
+    int64_t s_size;                 /* argv[5]  S (tuples)                            */
+    int32_t partition_size;         /* --partition_size: first-level fan-out P1 (power
+                                       of two, 2..2048); -1 = auto                     */
+    int32_t second_partition_size;  /* --second_partition_size: second-level fan-out
+                                       P2 (power of two, 1..2048); -1 = auto           */
+    int32_t radix_bits;             /* total radix bits log2(P1*P2) (1..15); -1 = from
+                                       the two fan-outs or auto                        */
+    int32_t use_bf;                 /* argv[8]: 0 = Bloom off; any supported nonzero
+                                       mode = Bloom on (one GPU: the filter built from
+                                       R is already complete, so every reference mode
+                                       collapses to "on")                              */
+    double  bf_bits_per_key;        /* > 0: filter bits per R key; <= 0: derive from
+                                       true_false_ratio with the reference formula     */
+    double  true_false_ratio;       /* argv[11] target false-positive rate p          */
+    int64_t num_bits;               /* --num_bits override (filter bits), -1 = none   */
+    int32_t num_hashes;             /* --num_hashes override (k, 1..8), -1 = none     */
+    double  answer_scale;           /* --answer_scale: row capacity =
+                                       max(R,S) * max(1, answer_scale)                */
+    int32_t tile;                   /* LDS build tile in tuples (4096 or 8192); 0=auto */
+    int32_t inputs_on_device;       /* 1: r_key.. are device pointers; 0: host         */
+    int32_t collect_phase_times;    /* 1: fill bj_result.phase_ms                      */
+    int32_t s_chunks;               /* argv[12]; accepted, S is processed in one pass  */
+} bj_params;
+
+/* ---- join result ------------------------------------------------------- */
+typedef struct bj_result {
+    uint64_t match_count;     /* exact number of (r, s) pairs with equal keys        */
+    uint64_t rows_written;    /* min(match_count, capacity)                          */
+    uint64_t capacity;        /* row capacity of the output buffers                  */
+    int32_t  truncated;       /* 1 if match_count > capacity (count stays exact)     */
+    uint64_t s_survivors;     /* |S'| after the Bloom filter (= S when it is off)    */
+    /* Device pointers to the rows (SoA), owned by the context; valid until the
+     * next bj_join on the same context or bj_destroy.  Row order is unspecified. */
+    uint32_t *key, *r_val, *s_val;
+    double   phase_ms[BJ_NUM_PHASES];
+    double   kernel_ms[BJ_NUM_KERNELS];
+    /* plan actually used */
+    int32_t  bits1, bits2;    /* first / second level radix bits                     */
+    int32_t  bf_seg_bits;     /* filter segments = 2^bf_seg_bits                     */
+    int32_t  bf_k;            /* hash bits per key                                   */
+    uint64_t bf_words;        /* filter size in 32-bit words (0 = Bloom off)         */
+    int32_t  tile;            /* LDS build tile (tuples)                             */
+    uint64_t heavy_items;     /* vlsplit work items for oversized sub-buckets        */
+} bj_result;
+
+/* ---- the plan a join would use (no device needed) ---------------------- */
+typedef struct bj_plan_info {
+    int32_t  bits1, bits2;       /* first / second level radix bits            */
+    int32_t  bf;                 /* Bloom on                                   */
+    int32_t  bf_seg_bits, bf_k;  /* filter segments = 2^seg_bits, bits per key */
+    uint64_t bf_words;           /* filter size in 32-bit words                */
+    int32_t  tile;               /* LDS build tile (tuples)                    */
+    uint64_t capacity;           /* output row capacity                        */
+    uint64_t workspace_bytes;    /* device memory a context needs for the join,
+                                    excluding device-resident inputs           */
+} bj_plan_info;
+
+typedef struct bj_ctx bj_ctx;
+
+/* ---- lifecycle --------------------------------------------------------- */
+int  bj_create(bj_ctx **ctx, int device);
+int  bj_destroy(bj_ctx *ctx);
+/* Run on an existing HIP stream (hipStream_t passed as void*); NULL = the
+ * context's own stream.  The stream must have the default priority. */
+int  bj_set_stream(bj_ctx *ctx, void *hip_stream);
+void bj_params_default(bj_params *p);
+const char *bj_strerror(int code);
+/* Detail of the last error raised on the calling host thread. */
+const char *bj_last_error(void);
+int  bj_version(void);
+
+/* Derive the plan (radix bits, filter geometry, capacity, device footprint)
+ * that bj_join would use for these parameters; validates them like bj_join.
+ * Counterpart of gpu_main's parameter derivation, src/hash_kernel.cu:174-284. */
+int  bj_plan(const bj_params *p, bj_plan_info *out);
+
+/* ---- the join ---------------------------------------------------------- */
+/* Equi-join R(key, r_val) with S(key, s_val) on key.  Emits every matching
+ * (key, r_val, s_val) row.  Sizes come from p->r_size / p->s_size. */
+int  bj_join(bj_ctx *ctx, const bj_params *p,
+             const uint32_t *r_key, const uint32_t *r_val,
+             const uint32_t *s_key, const uint32_t *s_val,
+             bj_result *out);
+
+/* Copy the first n rows of a result to host arrays (n <= rows_written). */
+int  bj_copy_rows_to_host(bj_ctx *ctx, const bj_result *res,
+                          uint32_t *key, uint32_t *r_val, uint32_t *s_val,
+                          uint64_t n);
+
+/* ---- device-side synthetic generator ------------------------------------
+ * R: key_i distinct in [1, 2^31) (a keyed 31-bit permutation of i), val_i = i.
+ * S: exactly m_s positions match (chosen by a keyed permutation of positions);
+ *    a matching key is R's key at a row drawn uniformly (zipf_s == 0) or with a
+ *    Zipf(zipf_s) law over R's rows (zipf_s > 0); every other key is distinct in
+ *    [2^31, 2^32).  val_j = j.  Pointers are device pointers. */
+int  bj_gen_build(bj_ctx *ctx, uint64_t seed, uint64_t n_r,
+                  uint32_t *d_key, uint32_t *d_val);
+int  bj_gen_probe(bj_ctx *ctx, uint64_t seed_r, uint64_t seed_s, uint64_t n_r,
+                  uint64_t n_s, uint64_t m_s, double zipf_s,
+                  uint32_t *d_key, uint32_t *d_val);
+
+/* ---- output.bin ----------------------------------------------------------
+ * Little-endian: char magic[8] = "BJOUT001", uint64 count, then uint32
+ * key[count], uint32 r_val[count], uint32 s_val[count] (SoA, mirroring the
+ * reference's GPU_answer / GPU_answer_val / GPU_answer_s_val,
+ * src/hash_kernel.cu:475-477).  bj_write_output sorts the rows by
+ * (key, r_val, s_val) first (in place), so two correct runs give byte-identical
+ * files. */
+int  bj_write_output(const char *path, uint64_t n,
+                     uint32_t *key, uint32_t *r_val, uint32_t *s_val);
+/* Reads a file written by bj_write_output; the arrays are malloc'ed and must be
+ * released with bj_free_host. */
+int  bj_read_output(const char *path, uint64_t *n,
+                    uint32_t **key, uint32_t **r_val, uint32_t **s_val);
+void bj_free_host(void *p);
+
+/* ---- the CPU join the reference prints beside the GPU count -------------
+ * Count of matching pairs by the reference's CPU algorithm (open addressing,
+ * capacity 2^ceil(log2 2R), multiplicative hash, linear probing).  Reported
+ * beside the GPU count by bj_gpu_main when sanity_check is set. */
+long long bj_cpu_sanity_count(const uint32_t *r_key, int64_t r_size,
+                              const uint32_t *s_key, int64_t s_size);
+
+/* ---- gpu_main, one-GPU share ---------------------------------------------
+ * Mirrors gpu_main's parameters (src/hash_kernel.h:21-34) and the driver's
+ * positional + flag surface (src/driver.cpp:43-198).  Multi-GPU-only knobs
+ * (use_nvlink, use_warp_compact, kernel_fusion, redist_skew_threshold,
+ * partition_rs_split, r_shuffle_enable, r_replicate_ratio) are accepted and
+ * have no effect on one GPU. */
+typedef struct bj_main_args {
+    const char *r_path;            /* argv[1] raw int32 keys (+ value block)     */
+    const char *s_path;            /* argv[2]                                    */
+    const char *out_path;          /* argv[3] output.bin (NULL or "" = none)     */
+    int64_t r_size, s_size;        /* argv[4], argv[5]                           */
+    int32_t sanity_check;          /* argv[6]                                    */
+    int32_t chunk_size;            /* argv[7] (accepted)                         */
+    int32_t use_bf;                /* argv[8]                                    */
+    int32_t use_nvlink;            /* argv[9]                                    */
+    int32_t use_warp_compact;      /* argv[10]                                   */
+    double  true_false_ratio;      /* argv[11]                                   */
+    int32_t s_chunks;              /* argv[12]                                   */
+    int32_t kernel_fusion;         /* argv[13]                                   */
+    double  redist_skew_threshold; /* argv[14]                                   */
+    int64_t override_num_bits;     /* --num_bits                                 */
+    int32_t override_num_hashes;   /* --num_hashes                               */
+    int32_t override_partition_rs_split;    /* --partition_rs_split            */
+    int32_t override_partition_size;        /* --partition_size                */
+    int32_t override_second_partition_size; /* --second_partition_size         */
+    int32_t fused_probe_breakdown; /* --fused_probe_breakdown: print phase times */
+    int32_t read_val;              /* --read_val: value block follows the keys   */
+    int32_t r_shuffle_enable;      /* --r_shuffle_enable                         */
+    double  r_replicate_ratio;     /* --r_replicate_ratio                        */
+    double  answer_scale;          /* --answer_scale                             */
+    /* additions of this library */
+    int32_t radix_bits;            /* --radix_bits (total), -1 = auto            */
+    double  bf_bits_per_key;       /* --bf_bits_per_key, <= 0 = formula          */
+    int32_t val_mode;              /* --val_mode: 0 = row index (default),
+                                      1 = the reference's i+1 / i+100000001      */
+    int32_t device;                /* --device                                   */
+    int32_t verbose;               /* print the reference-style summary          */
+} bj_main_args;
+
+void bj_main_args_default(bj_main_args *a);
+/* Returns BJ_OK and fills *res (rows stay on the device of the internal
+ * context, which is destroyed on return: only the counts remain valid). */
+int  bj_gpu_main(const bj_main_args *a, bj_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BJ_API_H */

@@ -1,0 +1,216 @@
+"""Parity of the HIP join against the oracle, through the C ABI (GPU tests).
+
+Bar: bit-exact.  The same match count and the same sorted multiset of
+(key, r_val, s_val) rows as the oracle's CPU join (restated from the reference's
+sanity_check and extended to rows) on the same inputs, and as the golden
+fixtures (brute-force rows, reference counts).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import brigjoin as bj
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+def to_dev(a):
+    import torch
+    a = np.ascontiguousarray(a, np.uint32)
+    if len(a) == 0:
+        return torch.zeros(1, dtype=torch.int32, device="cuda")
+    return torch.from_numpy(a.view(np.int32)).to("cuda")
+
+
+def dev_empty(n):
+    import torch
+    return torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+
+
+def to_host(t, n):
+    return t[:n].cpu().numpy().view(np.uint32)
+
+
+def gpu_join(ctx, rk, rv, sk, sv, **kw):
+    import torch
+    p = bj.default_params(r_size=len(rk), s_size=len(sk), inputs_on_device=1, **kw)
+    d = [to_dev(x) for x in (rk, rv, sk, sv)]
+    torch.cuda.synchronize()
+    print(f"[gpu_join] |R|={len(rk)} |S|={len(sk)} {kw}", flush=True)
+    res = ctx.join(p, *d)
+    k, a, b = ctx.rows_to_host(res)
+    return res, O.sort_rows(k, a, b)
+
+
+def oracle_rows(rk, rv, sk, sv):
+    n, k, a, b = O.cpu_join(rk, rv, sk, sv)
+    return n, O.sort_rows(k, a, b)
+
+
+VARIANTS = [
+    dict(),
+    dict(use_bf=1, bf_bits_per_key=10.0, num_hashes=3),
+    dict(radix_bits=1),
+    dict(radix_bits=4, tile=4096),
+    dict(partition_size=64, second_partition_size=16, use_bf=10),
+    dict(radix_bits=13, use_bf=1, bf_bits_per_key=4.0, num_hashes=2),
+]
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+@pytest.mark.parametrize("vi", range(len(VARIANTS)))
+def test_golden_fixtures(gpu_ctx, path, vi):
+    z = np.load(path, allow_pickle=False)
+    res, rows = gpu_join(gpu_ctx, z["r_key"], z["r_val"], z["s_key"], z["s_val"], **VARIANTS[vi])
+    assert res.match_count == int(z["count"])
+    assert res.truncated == 0 or int(z["count"]) > res.capacity
+    expect = z["rows"].reshape(-1, 3)
+    if not res.truncated:
+        assert np.array_equal(rows, expect)
+
+
+def test_generator_matches_oracle(gpu_ctx):
+    n_r, n_s = 1 << 20, 3 << 19
+    m = O.match_count(n_s, 0.1)
+    dk, dv = dev_empty(n_r), dev_empty(n_r)
+    gpu_ctx.gen_build(42, n_r, dk, dv)
+    rk, rv = O.gen_r(42, n_r)
+    assert np.array_equal(to_host(dk, n_r), rk) and np.array_equal(to_host(dv, n_r), rv)
+    sk_d, sv_d = dev_empty(n_s), dev_empty(n_s)
+    gpu_ctx.gen_probe(42, 42 ^ 0xDEADBEEF, n_r, n_s, m, sk_d, sv_d)
+    sk, sv = O.gen_s(42, 42 ^ 0xDEADBEEF, n_r, n_s, m)
+    assert np.array_equal(to_host(sk_d, n_s), sk) and np.array_equal(to_host(sv_d, n_s), sv)
+
+
+@pytest.mark.parametrize("n_r,n_s,ratio,kw", [
+    (1 << 20, 1 << 20, 0.1, dict()),
+    (1 << 20, 1 << 20, 0.1, dict(use_bf=1, bf_bits_per_key=10.0, num_hashes=3)),
+    (300_000, 2_000_000, 0.5, dict(use_bf=1, bf_bits_per_key=12.0, num_hashes=4)),
+    (2_000_000, 300_000, 1.0, dict(tile=4096)),
+    (1_000_000, 1_000_000, 0.1, dict(radix_bits=2)),           # vlsplit: chunks x slices
+    (77_777, 1_234_567, 0.3, dict(partition_size=32, radix_bits=9)),
+])
+def test_generated_parity(gpu_ctx, n_r, n_s, ratio, kw):
+    m = O.match_count(n_s, ratio)
+    rk, rv = O.gen_r(42, n_r)
+    sk, sv = O.gen_s(42, 42 ^ 0xDEADBEEF, n_r, n_s, m)
+    res, rows = gpu_join(gpu_ctx, rk, rv, sk, sv, **kw)
+    n, expect = oracle_rows(rk, rv, sk, sv)
+    assert n == m == res.match_count
+    assert np.array_equal(rows, expect)
+    if kw.get("use_bf"):
+        assert m <= res.s_survivors < n_s
+
+
+def test_config1_full_size(gpu_ctx):
+    """R=S=16M, 10% match, Bloom off, 10 radix bits: bit-exact against the oracle."""
+    import torch
+    n = 16_000_000
+    m = O.match_count(n, 0.1)
+    dk, dv, sk_d, sv_d = (dev_empty(n) for _ in range(4))
+    gpu_ctx.gen_build(42, n, dk, dv)
+    gpu_ctx.gen_probe(42, 42 ^ 0xDEADBEEF, n, n, m, sk_d, sv_d)
+    torch.cuda.synchronize()
+    p = bj.default_params(r_size=n, s_size=n, partition_size=1024, inputs_on_device=1)
+    res = gpu_ctx.join(p, dk, dv, sk_d, sv_d)
+    assert res.match_count == m == 1_600_000 and res.bits1 == 10
+    k, a, b = gpu_ctx.rows_to_host(res)
+    rk, rv = to_host(dk, n), to_host(dv, n)
+    sk, sv = to_host(sk_d, n), to_host(sv_d, n)
+    n_o, expect = oracle_rows(rk, rv, sk, sv)
+    assert n_o == m
+    assert np.array_equal(O.sort_rows(k, a, b), expect)
+
+
+def test_skewed_zipf_parity(gpu_ctx):
+    """Zipf(1.0) matching keys over R: heavy sub-buckets take the vlsplit path."""
+    import torch
+    n_r, n_s = 1 << 20, 12_000_000
+    m = O.match_count(n_s, 0.05)
+    dk, dv, sk_d, sv_d = dev_empty(n_r), dev_empty(n_r), dev_empty(n_s), dev_empty(n_s)
+    gpu_ctx.gen_build(7, n_r, dk, dv)
+    gpu_ctx.gen_probe(7, 9, n_r, n_s, m, sk_d, sv_d, zipf_s=1.0)
+    torch.cuda.synchronize()
+    sk = to_host(sk_d, n_s)
+    top = np.bincount(np.unique(sk[sk < 2**31], return_inverse=True)[1]).max()
+    assert top > 20_000                     # a genuinely heavy key
+    for kw in (dict(use_bf=1, bf_bits_per_key=12.0, num_hashes=4), dict()):
+        p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=1, **kw)
+        res = gpu_ctx.join(p, dk, dv, sk_d, sv_d)
+        k, a, b = gpu_ctx.rows_to_host(res)
+        rk, rv, sv = to_host(dk, n_r), to_host(dv, n_r), to_host(sv_d, n_s)
+        n_o, expect = oracle_rows(rk, rv, sk, sv)
+        assert res.match_count == n_o == m
+        assert np.array_equal(O.sort_rows(k, a, b), expect)
+
+
+def test_truncation_exact_count(gpu_ctx):
+    """Output beyond capacity is dropped, the count stays exact (many-to-many)."""
+    rk = np.full(3000, 7, np.uint32)
+    sk = np.concatenate([np.full(4000, 7, np.uint32), np.arange(100, 200, dtype=np.uint32)])
+    rv = np.arange(len(rk), dtype=np.uint32)
+    sv = np.arange(len(sk), dtype=np.uint32)
+    res, rows = gpu_join(gpu_ctx, rk, rv, sk, sv)
+    assert res.match_count == 3000 * 4000
+    assert res.truncated == 1 and res.rows_written == res.capacity == len(sk)
+    assert np.all(rows[:, 0] == 7)
+    assert np.all(rk[rows[:, 1]] == 7) and np.all(sk[rows[:, 2]] == 7)
+    assert len(np.unique(rows, axis=0)) == len(rows)
+    res2, rows2 = gpu_join(gpu_ctx, rk, rv, sk, sv, answer_scale=4000.0)
+    assert res2.truncated == 0 and res2.match_count == 12_000_000
+    assert len(np.unique(rows2[:, 1].astype(np.uint64) << np.uint64(32) | rows2[:, 2])) == 12_000_000
+
+
+def test_host_inputs_path(gpu_ctx):
+    n_r, n_s = 500_000, 700_000
+    m = O.match_count(n_s, 0.2)
+    rk, rv = O.gen_r(3, n_r)
+    sk, sv = O.gen_s(3, 4, n_r, n_s, m)
+    p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=0, use_bf=1)
+    res = gpu_ctx.join(p, rk, rv, sk, sv)
+    k, a, b = gpu_ctx.rows_to_host(res)
+    n, expect = oracle_rows(rk, rv, sk, sv)
+    assert res.match_count == n == m
+    assert np.array_equal(O.sort_rows(k, a, b), expect)
+
+
+def test_repeated_joins_reuse_context(gpu_ctx):
+    """The context keeps its workspace; a larger then a smaller join stay exact."""
+    for n, ratio in ((800_000, 0.3), (120_000, 0.9), (800_000, 0.01)):
+        m = O.match_count(n, ratio)
+        rk, rv = O.gen_r(n, n)
+        sk, sv = O.gen_s(n, n + 1, n, n, m)
+        res, rows = gpu_join(gpu_ctx, rk, rv, sk, sv, use_bf=1)
+        _, expect = oracle_rows(rk, rv, sk, sv)
+        assert res.match_count == m and np.array_equal(rows, expect)
+
+
+def test_gpu_main_reference_files(tmp_path):
+    """gpu_main's file surface: raw int32 key files (+ value block), output.bin."""
+    z = np.load(os.path.join(HERE, "golden", "refgen_2000.npz"), allow_pickle=False)
+    rpath, spath, opath = tmp_path / "R.bin", tmp_path / "S.bin", tmp_path / "output.bin"
+    z["r_key"].tofile(rpath)
+    z["s_key"].tofile(spath)
+    args = bj.main_args(r_path=str(rpath), s_path=str(spath), out_path=str(opath),
+                        r_size=len(z["r_key"]), s_size=len(z["s_key"]), sanity_check=1,
+                        use_bf=14, verbose=0)
+    res = bj.gpu_main(args)
+    assert res.match_count == int(z["count"]) == int(z["ref_count"])
+    k, a, b = bj.read_output(str(opath))
+    assert np.array_equal(np.stack([k, a, b], 1), z["rows"])
+    # --read_val: the value block follows the key block
+    rv = (np.arange(len(z["r_key"])) * 3 + 11).astype(np.uint32)
+    sv = (np.arange(len(z["s_key"])) * 5 + 7).astype(np.uint32)
+    np.concatenate([z["r_key"], rv]).tofile(rpath)
+    np.concatenate([z["s_key"], sv]).tofile(spath)
+    args = bj.main_args(r_path=str(rpath), s_path=str(spath), out_path=str(opath),
+                        r_size=len(rv), s_size=len(sv), read_val=1, verbose=0)
+    bj.gpu_main(args)
+    k, a, b = bj.read_output(str(opath))
+    n, expect = oracle_rows(z["r_key"], rv, z["s_key"], sv)
+    assert np.array_equal(np.stack([k, a, b], 1), expect)
