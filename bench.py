@@ -41,17 +41,22 @@ WORKLOADS = {
 
 
 def algorithmic_bytes(kernel: str, n_r: int, n_s: int, m: int, m_s: int, bf_bytes: int,
-                      bloom: bool) -> float:
+                      bloom: bool, bf_stage: int = 1) -> float:
     """Compulsory streamed bytes of one launch of a kernel:
     a partition pass reads and writes 8 B per tuple, build+probe read 8 B per R
     and per S' tuple and write 12 B per row, the Bloom build reads 4 B per R key
     and writes the filter once, the Bloom filter reads 8 B per S tuple and writes
-    8 B per matching S tuple; histogram passes read 4 B per key."""
+    8 B per matching S tuple; histogram passes read 4 B per key.  With the
+    filter after the first-level pass (bf_stage 2), S's histogram and first pass
+    cover all of S and the filter kernel also places the survivors (no second
+    pass over S')."""
     s_part = m_s if bloom else n_s
+    s_first = n_s if (bloom and bf_stage == 2) else s_part
+    s_second = 0.0 if (bloom and bf_stage == 2) else 16.0 * s_part
     return {
         "hist_r": 4.0 * n_r, "scatter1_r": 16.0 * n_r, "scatter2_r": 16.0 * n_r,
         "bf_build": 4.0 * n_r + bf_bytes, "bf_filter": 8.0 * n_s + 8.0 * m_s,
-        "hist_s": 4.0 * s_part, "scatter1_s": 16.0 * s_part, "scatter2_s": 16.0 * s_part,
+        "hist_s": 4.0 * s_first, "scatter1_s": 16.0 * s_first, "scatter2_s": s_second,
         "probe": 8.0 * n_r + 8.0 * s_part + 12.0 * m,
     }.get(kernel, 0.0)
 
@@ -112,6 +117,11 @@ def main() -> int:
     ap.add_argument("--workload", default="join640m_bf10", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--bf-stage", type=int, default=None, help="override: 1 before, 2 after pass 1")
+    ap.add_argument("--pass1-bits", type=int, default=None, help="override the first scatter pass bits")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="only run a plain device copy of this many GiB (calibrates rocprof "
+                         "FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -121,6 +131,16 @@ def main() -> int:
 
     import torch
     import brigjoin as bj
+
+    if args.copy_probe_gib > 0:
+        nbytes = int(args.copy_probe_gib * 2**30) // 4 * 4
+        src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
+        return 0
 
     n_r, n_s, ratio, zipf_s, params = WORKLOADS[args.workload]
     m_s = int(min(n_s, max(0, round(ratio * n_s))))
@@ -133,6 +153,10 @@ def main() -> int:
     ctx.gen_build(42, n_r, rk, rv)
     ctx.gen_probe(42, 42 ^ 0xDEADBEEF, n_r, n_s, m_s, sk, sv, zipf_s=zipf_s)
     torch.cuda.synchronize()
+    if args.bf_stage is not None:
+        params = dict(params, bf_stage=args.bf_stage)
+    if args.pass1_bits is not None:
+        params = dict(params, pass1_bits=args.pass1_bits)
     p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=1, **params)
     bloom = bool(params.get("use_bf"))
 
@@ -159,14 +183,14 @@ def main() -> int:
     bf_bytes = int(res.bf_words) * 4
     m = int(res.match_count)
     ms_sv = int(res.s_survivors)
-    dominant = max((k for k in kern_avg if algorithmic_bytes(k, n_r, n_s, m, m_s, bf_bytes, bloom) > 0),
-                   key=lambda k: kern_avg[k])
-    alg = algorithmic_bytes(dominant, n_r, n_s, m, m_s, bf_bytes, bloom)
+    stage = int(res.bf_stage)
+    ab = lambda k: algorithmic_bytes(k, n_r, n_s, m, m_s, bf_bytes, bloom, stage)  # noqa: E731
+    dominant = max((k for k in kern_avg if ab(k) > 0), key=lambda k: kern_avg[k])
+    alg = ab(dominant)
     achieved = alg / (kern_avg[dominant] * 1e-3) / 1e9
     traffic = load_traffic(args.workload, dominant)
     per_kernel = {k: {"ms": round(v, 4),
-                      "alg_GBps": round(algorithmic_bytes(k, n_r, n_s, m, m_s, bf_bytes, bloom)
-                                        / (v * 1e-3) / 1e9, 1) if v > 0 else None}
+                      "alg_GBps": round(ab(k) / (v * 1e-3) / 1e9, 1) if v > 0 else None}
                   for k, v in kern_avg.items() if v > 0}
     B = total_algorithmic_bytes(n_r, n_s, m, m_s, bf_bytes, bloom)
 
@@ -186,6 +210,7 @@ def main() -> int:
         "config": {"workload": args.workload, "n_r": n_r, "n_s": n_s, "match_ratio": ratio,
                    "zipf_s": zipf_s, **{k: v for k, v in params.items()},
                    "bits1": int(res.bits1), "bits2": int(res.bits2), "tile": int(res.tile),
+                   "bf_stage": int(res.bf_stage), "pass1_bits": int(res.pass1_bits),
                    "bf_seg_bits": int(res.bf_seg_bits), "bf_k": int(res.bf_k)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -28,7 +28,7 @@ EXPORTS = [
     "bj_create", "bj_destroy", "bj_set_stream", "bj_params_default", "bj_strerror",
     "bj_last_error", "bj_version", "bj_join", "bj_copy_rows_to_host", "bj_gen_build",
     "bj_gen_probe", "bj_write_output", "bj_read_output", "bj_free_host",
-    "bj_cpu_sanity_count", "bj_main_args_default", "bj_gpu_main", "bj_plan",
+    "bj_cpu_sanity_count", "bj_main_args_default", "bj_gpu_main", "bj_plan", "bj_partition",
 ]
 
 
@@ -37,6 +37,7 @@ class BjPlanInfo(ctypes.Structure):
         ("bits1", ctypes.c_int32), ("bits2", ctypes.c_int32), ("bf", ctypes.c_int32),
         ("bf_seg_bits", ctypes.c_int32), ("bf_k", ctypes.c_int32), ("bf_words", ctypes.c_uint64),
         ("tile", ctypes.c_int32), ("capacity", ctypes.c_uint64), ("workspace_bytes", ctypes.c_uint64),
+        ("bf_stage", ctypes.c_int32),
     ]
 
 
@@ -49,7 +50,8 @@ class BjParams(ctypes.Structure):
         ("num_bits", ctypes.c_int64), ("num_hashes", ctypes.c_int32),
         ("answer_scale", ctypes.c_double), ("tile", ctypes.c_int32),
         ("inputs_on_device", ctypes.c_int32), ("collect_phase_times", ctypes.c_int32),
-        ("s_chunks", ctypes.c_int32),
+        ("s_chunks", ctypes.c_int32), ("bf_stage", ctypes.c_int32),
+        ("pass1_bits", ctypes.c_int32),
     ]
 
 
@@ -63,7 +65,8 @@ class BjResult(ctypes.Structure):
         ("kernel_ms", ctypes.c_double * 12),
         ("bits1", ctypes.c_int32), ("bits2", ctypes.c_int32), ("bf_seg_bits", ctypes.c_int32),
         ("bf_k", ctypes.c_int32), ("bf_words", ctypes.c_uint64), ("tile", ctypes.c_int32),
-        ("heavy_items", ctypes.c_uint64),
+        ("heavy_items", ctypes.c_uint64), ("bf_stage", ctypes.c_int32),
+        ("pass1_bits", ctypes.c_int32),
     ]
 
     def phases(self) -> dict:
@@ -141,6 +144,7 @@ def lib():
         L.bj_main_args_default.restype = None
         L.bj_gpu_main.argtypes = [ctypes.POINTER(BjMainArgs), ctypes.POINTER(BjResult)]
         L.bj_plan.argtypes = [ctypes.POINTER(BjParams), ctypes.POINTER(BjPlanInfo)]
+        L.bj_partition.argtypes = [vp, ctypes.POINTER(BjParams), vp, vp, ctypes.c_int64, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -176,7 +180,7 @@ def _ptr(x):
         assert x.dtype == np.uint32 and x.flags["C_CONTIGUOUS"]
         return x.ctypes.data
     # torch tensor
-    assert x.is_contiguous() and x.element_size() == 4
+    assert x.is_contiguous()
     return x.data_ptr()
 
 
@@ -213,6 +217,10 @@ class Context:
                            _ptr(s_key), _ptr(s_val), ctypes.byref(res))
         check(rc, "bj_join")
         return res
+
+    def partition(self, params: BjParams, key, val, n: int, out_key, out_val, out_start) -> None:
+        check(lib().bj_partition(self.h, ctypes.byref(params), _ptr(key), _ptr(val), n,
+                                 _ptr(out_key), _ptr(out_val), _ptr(out_start)), "bj_partition")
 
     def rows_to_host(self, res: BjResult, n: int | None = None):
         n = res.rows_written if n is None else n

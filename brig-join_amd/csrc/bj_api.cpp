@@ -49,9 +49,8 @@ struct DevBuf {
 
 enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_N = 8 };
 
-// Event i is recorded when kernel slot i - 1 ends (slot order of BJ_K_*);
-// EV_K0 is recorded before the first hot-path kernel.
-enum { EV_H2D0 = 0, EV_H2D1, EV_K0, EV_N = EV_K0 + BJ_NUM_KERNELS + 1 };
+// Kernel slot k (BJ_K_*) is bracketed by events EV_KB + k and EV_KE + k.
+enum { EV_H2D0 = 0, EV_H2D1, EV_KB, EV_KE = EV_KB + BJ_NUM_KERNELS, EV_N = EV_KE + BJ_NUM_KERNELS };
 
 uint64_t splitmix64(uint64_t x)
 {
@@ -96,7 +95,8 @@ struct bj_ctx {
     DevBuf in_rk, in_rv, in_sk, in_sv;
     // partitioned relations and scratch
     DevBuf r_key, r_val, s_key, s_val, tmp_key, tmp_val, sp_key, sp_val;
-    DevBuf filter, part, totals, l1part, l1off, r_start, s_start, partials, items, counters;
+    DevBuf filter, part, totals, l1h, l1off, l2cur, r_start, s_start, s_fill, partials, items, counters;
+    DevBuf plan_cnt, item_start;
     DevBuf out_key, out_rval, out_sval, dtmp;
     uint64_t* h_counters = nullptr;  // pinned
 };
@@ -140,11 +140,15 @@ T* P(DevBuf& b) { return (T*)b.p; }
 // ----------------------------------------------------------------- the plan
 struct Plan {
     uint64_t nr = 0, ns = 0;
-    int bits = 1, bits1 = 1, bits2 = 0;
+    int bits = 1, bits1 = 1, bits2 = 0;  // logical levels (partition_size, second level)
+    int p1 = 1;                          // bits of the first physical scatter pass
+    int W2 = 1;                          // second-pass workgroups per first-pass bucket
     int G_r = 1, G_s = 1;
     int tile = 8192;
     uint32_t slice = 65536;
+    uint32_t max_slices = 16;
     bool bf = false;
+    int bf_stage = 0;
     BloomGeom geom{0, 8, 1};
     uint64_t bf_words = 0;
     uint64_t cap = 1;
@@ -156,11 +160,12 @@ constexpr int MAX_BITS = 15;                     // joint histogram in LDS: 2^15
 constexpr int MAX_LEVEL_BITS = 11;               // fan-out per scatter pass
 constexpr uint64_t SEG_TARGET_BYTES = 64 * 1024; // Bloom segment built in LDS
 constexpr uint64_t SEG_MAX_BYTES = 160 * 1024;
+constexpr uint64_t MALL_FILTER_BYTES = 128ull << 20;  // filters up to half the MALL probe well unpartitioned
 
 int choose_G(uint64_t n)
 {
-    uint64_t g = (n + 16383) / 16384;
-    return (int)std::max<uint64_t>(1, std::min<uint64_t>(1024, g));
+    uint64_t g = (n + 32767) / 32768;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(512, g));
 }
 
 void split_bits(int total, int& b1, int& b2)
@@ -265,62 +270,83 @@ int make_plan(const bj_params& p, Plan& pl)
     pl.bits = total;
     pl.bits1 = b1;
     pl.bits2 = b2;
+    if (p.pass1_bits > 0) {
+        if (p.pass1_bits > total) return fail(BJ_ERR_INVALID, "pass1_bits exceeds the total radix bits");
+        pl.p1 = p.pass1_bits;
+    } else {
+        pl.p1 = total <= 7 ? total : (total + 1) / 2;
+    }
+    if (pl.p1 > MAX_LEVEL_BITS || total - pl.p1 > MAX_LEVEL_BITS)
+        return fail(BJ_ERR_SIZE, "a scatter pass exceeds 2048-way fan-out");
+    // enough second-pass workgroups to fill the chip: about 1024 in total
+    pl.W2 = std::max(1, std::min(64, 1024 >> pl.p1));
 
     if (pl.bf) {
         int seg_bits = std::min(seg_need, total);
-        uint64_t words = (bf_bytes + 3) / 4;
-        uint64_t wps = (words + (1ull << seg_bits) - 1) >> seg_bits;
-        wps = std::max<uint64_t>(8, (wps + 7) / 8 * 8);
-        if (wps * 4 > SEG_MAX_BYTES)
+        const uint64_t blocks = (bf_bytes + 7) / 8;  // 64-bit blocks
+        uint64_t wps = std::max<uint64_t>(1, (blocks + (1ull << seg_bits) - 1) >> seg_bits);
+        if (wps * 8 > SEG_MAX_BYTES)
             return fail(BJ_ERR_SIZE, "Bloom segment exceeds LDS: raise radix bits or lower filter bits");
         pl.geom.seg_bits = seg_bits;
         pl.geom.wps = (uint32_t)wps;
         pl.geom.k = k;
-        pl.bf_words = wps << seg_bits;
+        pl.bf_words = (wps << seg_bits) * 2;  // in 32-bit words
+        if (p.bf_stage < 0 || p.bf_stage > 2) return fail(BJ_ERR_INVALID, "bf_stage must be 0, 1 or 2");
+        pl.bf_stage = p.bf_stage ? p.bf_stage : (pl.bf_words * 4 <= MALL_FILTER_BYTES ? 1 : 2);
     }
 
     pl.G_r = choose_G(pl.nr);
     pl.G_s = choose_G(pl.ns);
+    pl.W2 = std::min(pl.W2, std::min(pl.G_r, pl.G_s));
     const double scale = std::max(1.0, p.answer_scale);
     pl.cap = std::max<uint64_t>(1, (uint64_t)((double)std::max(pl.nr, pl.ns) * scale));
-    pl.item_cap = (1ull << total) + (pl.nr + pl.ns) / pl.tile + (pl.nr + pl.ns) / pl.slice + 1024;
+    // sum over sub-buckets of ceil(nb / tile) <= (nr + ns) / tile + 2^bits
+    pl.item_cap = (uint64_t)pl.max_slices * ((pl.nr + pl.ns) / pl.tile + (1ull << total)) + 16;
     return BJ_OK;
 }
 
-// Two-level partition of one relation into out_key/out_val with sub-bucket
-// starts in start[0 .. 2^bits].
+// Radix partition of one relation into out_key/out_val with sub-bucket starts
+// in start[0 .. 2^bits], in one or two physical scatter passes (pl.p1 bits,
+// then the rest).  With l1_only, only the first pass runs and its output
+// (grouped by first-pass bucket, in the final positions' ranges) is left in
+// out_key/out_val.
 int run_partition(bj_ctx* c, const Plan& pl, const uint32_t* key, const uint32_t* val,
                   uint64_t n_host, const uint64_t* n_dev, int G, uint32_t* out_key,
-                  uint32_t* out_val, uint64_t out_cap, uint64_t* start, int k_hist)
+                  uint32_t* out_val, uint64_t out_cap, uint64_t* start, int k_hist, bool l1_only = false)
 {
     // k_hist is the BJ_K_HIST_* slot; the three slots after it follow in order
-    auto mark = [&](int slot) { return hipEventRecord(c->ev[EV_K0 + 1 + slot], c->stream); };
+    auto beg = [&](int slot) { return hipEventRecord(c->ev[EV_KB + slot], c->stream); };
+    auto end = [&](int slot) { return hipEventRecord(c->ev[EV_KE + slot], c->stream); };
     hipStream_t st = c->stream;
     const int nbins = 1 << pl.bits;
-    const uint64_t n_l1 = (uint64_t)(1 << pl.bits1) * G;
+    const int p1 = pl.p1, b2 = pl.bits - pl.p1;
+    const int P1 = 1 << p1;
     uint64_t* err = P<uint64_t>(c->counters) + CTR_ERR;
-    BJ_HIP(launch_hist(key, n_host, n_dev, G, pl.bits, P<uint32_t>(c->part), st));
-    BJ_HIP(mark(k_hist));
-    BJ_HIP(launch_reduce(P<uint32_t>(c->part), G, pl.bits, pl.bits2, P<uint32_t>(c->totals),
-                         P<uint32_t>(c->l1part), st));
+    uint32_t* part = P<uint32_t>(c->part);
+    BJ_HIP(beg(k_hist));
+    BJ_HIP(launch_hist(key, n_host, n_dev, G, pl.bits, part, st));
+    BJ_HIP(end(k_hist));
+    BJ_HIP(beg(k_hist + 1));
+    BJ_HIP(launch_reduce(part, G, pl.bits, b2, P<uint32_t>(c->totals), P<uint32_t>(c->l1h), st));
     BJ_HIP(launch_scan(P<uint32_t>(c->totals), (uint64_t)nbins, start, P<uint64_t>(c->partials), st));
-    BJ_HIP(launch_scan(P<uint32_t>(c->l1part), n_l1, P<uint64_t>(c->l1off), P<uint64_t>(c->partials), st));
-    BJ_HIP(mark(k_hist + 1));
-    if (pl.bits2 > 0) {
-        BJ_HIP(launch_scatter(key, val, P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val),
-                              c->tmp_key.bytes / 4, n_host, n_dev, G, 0, pl.bits, pl.bits2,
-                              P<uint64_t>(c->l1off), start, err, st));
-        BJ_HIP(mark(k_hist + 2));
-        BJ_HIP(launch_scatter(P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val), out_key, out_val, out_cap,
-                              n_host, n_dev, G, 1, pl.bits, pl.bits2, P<uint64_t>(c->l1off), start, err,
-                              st));
-        BJ_HIP(mark(k_hist + 3));
-    } else {
-        BJ_HIP(launch_scatter(key, val, out_key, out_val, out_cap, n_host, n_dev, G, 0, pl.bits, 0,
-                              P<uint64_t>(c->l1off), start, err, st));
-        BJ_HIP(mark(k_hist + 2));
-        BJ_HIP(mark(k_hist + 3));
-    }
+    const uint32_t* l1h = b2 > 0 ? P<uint32_t>(c->l1h) : part;
+    BJ_HIP(launch_l1_offsets(l1h, G, P1, b2, start, P<uint64_t>(c->l1off), st));
+    const bool two_pass = b2 > 0 && !l1_only;
+    const int W2 = std::min(pl.W2, G);
+    if (two_pass) BJ_HIP(launch_l2_offsets(part, G, pl.bits, W2, start, P<uint64_t>(c->l2cur), st));
+    BJ_HIP(end(k_hist + 1));
+    uint32_t* k1 = two_pass ? P<uint32_t>(c->tmp_key) : out_key;
+    uint32_t* v1 = two_pass ? P<uint32_t>(c->tmp_val) : out_val;
+    const uint64_t cap1 = two_pass ? c->tmp_key.bytes / 4 : out_cap;
+    BJ_HIP(beg(k_hist + 2));
+    BJ_HIP(launch_scatter(key, val, k1, v1, cap1, n_host, n_dev, G, 0, pl.bits, p1, W2,
+                          P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, st));
+    BJ_HIP(end(k_hist + 2));
+    BJ_HIP(beg(k_hist + 3));
+    if (two_pass)
+        BJ_HIP(launch_scatter(k1, v1, out_key, out_val, out_cap, n_host, n_dev, G, 1, pl.bits, p1, W2,
+                              P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, st));
+    BJ_HIP(end(k_hist + 3));
     return BJ_OK;
 }
 
@@ -338,24 +364,30 @@ int ensure_workspace(bj_ctx* c, const Plan& pl, uint64_t* total_bytes = nullptr)
     const uint64_t nr = std::max<uint64_t>(pl.nr, 1), ns = std::max<uint64_t>(pl.ns, 1);
     const uint64_t nbins = 1ull << pl.bits;
     const uint64_t gmax = (uint64_t)std::max(pl.G_r, pl.G_s);
-    const uint64_t n_l1 = (1ull << pl.bits1) * gmax;
+    const uint64_t n_l1 = (1ull << pl.p1) * gmax;
     if ((rc = need(B(&bj_ctx::r_key), nr * 4)) || (rc = need(B(&bj_ctx::r_val), nr * 4))) return rc;
     if ((rc = need(B(&bj_ctx::s_key), ns * 4)) || (rc = need(B(&bj_ctx::s_val), ns * 4))) return rc;
-    if (pl.bits2 > 0) {
+    if (pl.p1 < pl.bits || (pl.bf && pl.bf_stage == 2)) {
         const uint64_t t = std::max(nr, ns);
         if ((rc = need(B(&bj_ctx::tmp_key), t * 4)) || (rc = need(B(&bj_ctx::tmp_val), t * 4))) return rc;
     }
     if (pl.bf) {
-        if ((rc = need(B(&bj_ctx::sp_key), ns * 4)) || (rc = need(B(&bj_ctx::sp_val), ns * 4))) return rc;
+        if (pl.bf_stage == 1) {
+            if ((rc = need(B(&bj_ctx::sp_key), ns * 4)) || (rc = need(B(&bj_ctx::sp_val), ns * 4))) return rc;
+        }
         if ((rc = need(B(&bj_ctx::filter), pl.bf_words * 4))) return rc;
+        if ((rc = need(B(&bj_ctx::s_fill), nbins * 4))) return rc;
     }
     if ((rc = need(B(&bj_ctx::part), gmax * nbins * 4))) return rc;
     if ((rc = need(B(&bj_ctx::totals), nbins * 4))) return rc;
-    if ((rc = need(B(&bj_ctx::l1part), n_l1 * 4))) return rc;
-    if ((rc = need(B(&bj_ctx::l1off), (n_l1 + 1) * 8))) return rc;
+    if ((rc = need(B(&bj_ctx::l1h), n_l1 * 4))) return rc;
+    if ((rc = need(B(&bj_ctx::l1off), n_l1 * 8))) return rc;
+    if ((rc = need(B(&bj_ctx::l2cur), (uint64_t)pl.W2 * nbins * 8))) return rc;
+    if ((rc = need(B(&bj_ctx::plan_cnt), nbins * 4))) return rc;
+    if ((rc = need(B(&bj_ctx::item_start), (nbins + 1) * 8))) return rc;
     if ((rc = need(B(&bj_ctx::r_start), (nbins + 1) * 8)) || (rc = need(B(&bj_ctx::s_start), (nbins + 1) * 8)))
         return rc;
-    if ((rc = need(B(&bj_ctx::partials), scan_partials_needed(std::max(nbins, n_l1)) * 8 + 64))) return rc;
+    if ((rc = need(B(&bj_ctx::partials), scan_partials_needed(nbins) * 8 + 64))) return rc;
     if ((rc = need(B(&bj_ctx::items), pl.item_cap * sizeof(HeavyItem)))) return rc;
     if ((rc = need(B(&bj_ctx::out_key), pl.cap * 4)) || (rc = need(B(&bj_ctx::out_rval), pl.cap * 4)) ||
         (rc = need(B(&bj_ctx::out_sval), pl.cap * 4)))
@@ -415,6 +447,8 @@ void bj_params_default(bj_params* p)
     p->inputs_on_device = 0;
     p->collect_phase_times = 1;
     p->s_chunks = 1;
+    p->bf_stage = 0;
+    p->pass1_bits = -1;
 }
 
 int bj_create(bj_ctx** out, int device)
@@ -440,7 +474,7 @@ int bj_create(bj_ctx** out, int device)
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = kernels_init(max_lds);
     for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_counters, CTR_N * sizeof(uint64_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_counters, (CTR_N + 1) * sizeof(uint64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         bj_destroy(c);
         return fail(BJ_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
@@ -463,9 +497,9 @@ int bj_destroy(bj_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->in_rk,  &c->in_rv,    &c->in_sk,   &c->in_sv,    &c->r_key,   &c->r_val,
                       &c->s_key,  &c->s_val,    &c->tmp_key, &c->tmp_val,  &c->sp_key,  &c->sp_val,
-                      &c->filter, &c->part,     &c->totals,  &c->l1part,   &c->l1off,   &c->r_start,
-                      &c->s_start, &c->partials, &c->items,  &c->counters, &c->out_key, &c->out_rval,
-                      &c->out_sval, &c->dtmp};
+                      &c->filter, &c->part,     &c->totals,  &c->l1h,      &c->l1off,   &c->l2cur,   &c->r_start,
+                      &c->s_start, &c->s_fill,  &c->partials, &c->items,  &c->counters, &c->out_key,
+                      &c->out_rval, &c->out_sval, &c->dtmp,  &c->plan_cnt, &c->item_start};
     for (DevBuf* b : bufs) release(*b);
     for (int i = 0; i < EV_N; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -501,6 +535,7 @@ int bj_plan(const bj_params* p, bj_plan_info* out)
     out->tile = pl.tile;
     out->capacity = pl.cap;
     out->workspace_bytes = ws;
+    out->bf_stage = pl.bf ? pl.bf_stage : 0;
     return BJ_OK;
 }
 
@@ -541,8 +576,12 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
 
     uint64_t* ctr = P<uint64_t>(c->counters);
     BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
-    auto mark = [&](int slot) { return hipEventRecord(c->ev[EV_K0 + 1 + slot], st); };
-    BJ_HIP(hipEventRecord(c->ev[EV_K0], st));
+    auto beg = [&](int slot) { return hipEventRecord(c->ev[EV_KB + slot], st); };
+    auto end = [&](int slot) { return hipEventRecord(c->ev[EV_KE + slot], st); };
+    auto skip = [&](int slot) {
+        hipError_t e = beg(slot);
+        return e == hipSuccess ? end(slot) : e;
+    };
 
     // R: two-level partition
     if ((rc = run_partition(c, pl, r_key, r_val, pl.nr, nullptr, pl.G_r, P<uint32_t>(c->r_key),
@@ -550,39 +589,67 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
                             BJ_K_HIST_R)))
         return rc;
 
+    const uint32_t* s_cnt = nullptr;
     if (pl.bf) {
+        BJ_HIP(beg(BJ_K_BF_BUILD));
         BJ_HIP(launch_bloom_build(P<uint32_t>(c->r_key), P<uint64_t>(c->r_start), pl.bits, pl.geom,
-                                  P<uint32_t>(c->filter), st));
-        BJ_HIP(mark(BJ_K_BF_BUILD));
-        BJ_HIP(launch_bloom_filter(s_key, s_val, pl.ns, P<uint32_t>(c->filter), pl.geom,
-                                   P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), c->sp_key.bytes / 4,
-                                   ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
-        BJ_HIP(mark(BJ_K_BF_FILTER));
-        if ((rc = run_partition(c, pl, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns,
-                                ctr + CTR_SPRIME, pl.G_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val),
-                                c->s_key.bytes / 4, P<uint64_t>(c->s_start), BJ_K_HIST_S)))
-            return rc;
+                                  P<uint64_t>(c->filter), st));
+        BJ_HIP(end(BJ_K_BF_BUILD));
+        if (pl.bf_stage == 1) {
+            // the reference's order: filter S, then partition the survivors
+            BJ_HIP(beg(BJ_K_BF_FILTER));
+            BJ_HIP(launch_bloom_prefilter(s_key, s_val, pl.ns, P<uint64_t>(c->filter), pl.geom,
+                                          P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val),
+                                          c->sp_key.bytes / 4, ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
+            BJ_HIP(end(BJ_K_BF_FILTER));
+            if ((rc = run_partition(c, pl, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns,
+                                    ctr + CTR_SPRIME, pl.G_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val),
+                                    c->s_key.bytes / 4, P<uint64_t>(c->s_start), BJ_K_HIST_S)))
+                return rc;
+        } else {
+            // first-level pass of S, then filter each bucket against its L2-resident
+            // filter slice while placing the survivors into their sub-buckets
+            if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->tmp_key),
+                                    P<uint32_t>(c->tmp_val), c->tmp_key.bytes / 4, P<uint64_t>(c->s_start),
+                                    BJ_K_HIST_S, true)))
+                return rc;
+            BJ_HIP(beg(BJ_K_BF_FILTER));
+            BJ_HIP(launch_zero_u32(P<uint32_t>(c->s_fill), 1ull << pl.bits, st));
+            BJ_HIP(launch_filter_l1(P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val), P<uint64_t>(c->s_start),
+                                    pl.bits, pl.bits - pl.p1, P<uint64_t>(c->filter), pl.geom,
+                                    P<uint32_t>(c->s_key),
+                                    P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint32_t>(c->s_fill),
+                                    ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
+            s_cnt = P<uint32_t>(c->s_fill);
+            BJ_HIP(end(BJ_K_BF_FILTER));
+        }
     } else {
-        BJ_HIP(mark(BJ_K_BF_BUILD));
-        BJ_HIP(mark(BJ_K_BF_FILTER));
+        BJ_HIP(skip(BJ_K_BF_BUILD));
+        BJ_HIP(skip(BJ_K_BF_FILTER));
         if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->s_key),
                                 P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint64_t>(c->s_start),
                                 BJ_K_HIST_S)))
             return rc;
     }
 
-    BJ_HIP(launch_plan_probe(P<uint64_t>(c->r_start), P<uint64_t>(c->s_start), pl.bits, (uint32_t)pl.tile,
-                             pl.slice, P<HeavyItem>(c->items), pl.item_cap, ctr + CTR_ITEMS,
-                             ctr + CTR_ERR, st));
-    BJ_HIP(mark(BJ_K_PLAN));
+    const int nbins = 1 << pl.bits;
+    uint64_t* item_start = P<uint64_t>(c->item_start);
+    BJ_HIP(beg(BJ_K_PLAN));
+    BJ_HIP(launch_plan(P<uint64_t>(c->r_start), P<uint64_t>(c->s_start), s_cnt, pl.bits, (uint32_t)pl.tile,
+                       pl.slice, pl.max_slices, P<uint32_t>(c->plan_cnt), item_start, P<uint64_t>(c->partials),
+                       P<HeavyItem>(c->items), pl.item_cap, ctr + CTR_ERR, st));
+    BJ_HIP(end(BJ_K_PLAN));
+    BJ_HIP(beg(BJ_K_PROBE));
     BJ_HIP(launch_probe(pl.tile, P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start),
-                        P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), P<uint64_t>(c->s_start), pl.bits,
-                        pl.slice, P<HeavyItem>(c->items), ctr + CTR_ITEMS, ctr + CTR_WORK, ctr + CTR_OUT,
+                        P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), P<uint64_t>(c->s_start), s_cnt, pl.bits,
+                        pl.slice, P<HeavyItem>(c->items), item_start + nbins, pl.item_cap, ctr + CTR_WORK,
+                        ctr + CTR_OUT,
                         pl.cap, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
                         P<uint32_t>(c->out_sval), c->num_cu, st));
-    BJ_HIP(mark(BJ_K_PROBE));
+    BJ_HIP(end(BJ_K_PROBE));
 
     BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    BJ_HIP(hipMemcpyAsync(c->h_counters + CTR_N, item_start + nbins, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     BJ_HIP(hipStreamSynchronize(st));
 
     const uint64_t* h = c->h_counters;
@@ -601,18 +668,46 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
     out->bf_k = pl.bf ? pl.geom.k : 0;
     out->bf_words = pl.bf_words;
     out->tile = pl.tile;
-    out->heavy_items = h[CTR_ITEMS];
+    out->heavy_items = h[CTR_N];
+    out->bf_stage = pl.bf ? pl.bf_stage : 0;
+    out->pass1_bits = pl.p1;
     if (p->collect_phase_times) {
-        for (int k = 0; k < BJ_NUM_KERNELS; k++) out->kernel_ms[k] = ev_ms(c, EV_K0 + k, EV_K0 + k + 1);
-        auto span = [&](int k_first, int k_last) { return ev_ms(c, EV_K0 + k_first, EV_K0 + k_last + 1); };
+        double* k = out->kernel_ms;
+        for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = ev_ms(c, EV_KB + i, EV_KE + i);
         out->phase_ms[BJ_PH_H2D] = ev_ms(c, EV_H2D0, EV_H2D1);
-        out->phase_ms[BJ_PH_R_PART] = span(BJ_K_HIST_R, BJ_K_SCATTER2_R);
-        out->phase_ms[BJ_PH_BF_BUILD] = span(BJ_K_BF_BUILD, BJ_K_BF_BUILD);
-        out->phase_ms[BJ_PH_S_FILTER] = span(BJ_K_BF_FILTER, BJ_K_BF_FILTER);
-        out->phase_ms[BJ_PH_S_PART] = span(BJ_K_HIST_S, BJ_K_SCATTER2_S);
-        out->phase_ms[BJ_PH_PROBE] = span(BJ_K_PLAN, BJ_K_PROBE);
-        out->phase_ms[BJ_PH_TOTAL] = span(BJ_K_HIST_R, BJ_K_PROBE);
+        out->phase_ms[BJ_PH_R_PART] = k[BJ_K_HIST_R] + k[BJ_K_SCAN_R] + k[BJ_K_SCATTER1_R] + k[BJ_K_SCATTER2_R];
+        out->phase_ms[BJ_PH_BF_BUILD] = k[BJ_K_BF_BUILD];
+        out->phase_ms[BJ_PH_S_FILTER] = k[BJ_K_BF_FILTER];
+        out->phase_ms[BJ_PH_S_PART] = k[BJ_K_HIST_S] + k[BJ_K_SCAN_S] + k[BJ_K_SCATTER1_S] + k[BJ_K_SCATTER2_S];
+        out->phase_ms[BJ_PH_PROBE] = k[BJ_K_PLAN] + k[BJ_K_PROBE];
+        out->phase_ms[BJ_PH_TOTAL] = ev_ms(c, EV_KB + BJ_K_HIST_R, EV_KE + BJ_K_PROBE);
     }
+    return BJ_OK;
+}
+
+int bj_partition(bj_ctx* c, const bj_params* p, const uint32_t* key, const uint32_t* val, int64_t n,
+                 uint32_t* out_key, uint32_t* out_val, uint64_t* out_start)
+{
+    if (!c || !p || !out_start || (n > 0 && (!key || !val || !out_key || !out_val)))
+        return fail(BJ_ERR_INVALID, "null argument");
+    bj_params q = *p;
+    q.r_size = n;
+    q.s_size = n;
+    q.use_bf = 0;
+    Plan pl;
+    int rc = make_plan(q, pl);
+    if (rc) return rc;
+    BJ_HIP(hipSetDevice(c->device));
+    if ((rc = ensure_workspace(c, pl))) return rc;
+    hipStream_t st = c->stream;
+    uint64_t* ctr = P<uint64_t>(c->counters);
+    BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
+    if ((rc = run_partition(c, pl, key, val, pl.nr, nullptr, pl.G_r, out_key, out_val, pl.nr, out_start,
+                            BJ_K_HIST_R)))
+        return rc;
+    BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    BJ_HIP(hipStreamSynchronize(st));
+    if (c->h_counters[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(c->h_counters[CTR_ERR]));
     return BJ_OK;
 }
 

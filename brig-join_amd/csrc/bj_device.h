@@ -56,43 +56,38 @@ __host__ __device__ __forceinline__ uint32_t gen_r_key(uint32_t i, const PermKey
 }
 
 // ---------------------------------------------------------------- Bloom
-// Blocked filter: 2^seg_bits segments of `wps` words (a multiple of 8); a key's
-// k bits all live in one 256-bit block of its segment.  The segment is the top
+// Register-blocked filter: 2^seg_bits segments of `wps` 64-bit blocks; all k
+// bits of a key live in one 64-bit block of its segment, so a build is one
+// 64-bit LDS atomic OR and a query is one 8-byte load.  The segment is the top
 // seg_bits of the radix hash, so segment s holds exactly the R keys of the
 // sub-buckets [s << (bits-seg_bits), (s+1) << (bits-seg_bits)).
 struct BloomGeom {
     int seg_bits;
-    uint32_t wps;   // words per segment
+    uint32_t wps;   // 64-bit blocks per segment
     int k;          // bits per key, 1..8
 };
 
-struct BloomAddr {
-    uint64_t block_word;  // first word of the 256-bit block
-    uint32_t mask[8];     // per-word bit mask inside the block
-};
-
-__device__ __forceinline__ void bloom_addr(uint32_t key, const BloomGeom& g, BloomAddr& a)
+__host__ __device__ __forceinline__ uint64_t bloom_block(uint32_t key, const BloomGeom& g)
 {
-    const uint32_t h = radix_hash(key);
+    const uint32_t h = lowbias32(key);
     const uint64_t seg = g.seg_bits ? (uint64_t)(h >> (32 - g.seg_bits)) : 0ull;
     const uint32_t rest = g.seg_bits ? (h << g.seg_bits) : h;
-    const uint64_t nblk = g.wps >> 3;
-    const uint64_t blk = ((uint64_t)rest * nblk) >> 32;
-    a.block_word = seg * g.wps + blk * 8ull;
+    return seg * g.wps + (((uint64_t)rest * g.wps) >> 32);
+}
+
+__host__ __device__ __forceinline__ uint64_t bloom_mask(uint32_t key, int k)
+{
     const uint32_t h2 = lowbias32(key ^ 0x9E3779B9u);
     const uint32_t h3 = lowbias32(h2 ^ 0x85EBCA6Bu);
-#pragma unroll
-    for (int w = 0; w < 8; w++) a.mask[w] = 0u;
+    uint64_t m = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        if (i < g.k) {
-            const uint32_t bit = (i < 4) ? ((h2 >> (8 * i)) & 255u) : ((h3 >> (8 * (i - 4))) & 255u);
-            // select the word without dynamic register indexing
-#pragma unroll
-            for (int w = 0; w < 8; w++)
-                a.mask[w] |= ((bit >> 5) == (uint32_t)w) ? (1u << (bit & 31u)) : 0u;
+        if (i < k) {
+            const uint32_t bit = (i < 5) ? ((h2 >> (6 * i)) & 63u) : ((h3 >> (6 * (i - 5))) & 63u);
+            m |= 1ull << bit;
         }
     }
+    return m;
 }
 
 // ---------------------------------------------------------------- wave64 scans

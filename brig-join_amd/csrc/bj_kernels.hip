@@ -3,10 +3,11 @@
 // Pipeline (one GPU; all integer, all exact):
 //   R:  k_hist -> k_reduce_hist -> scans -> k_scatter(pass 1) -> k_scatter(pass 2)
 //   BF: k_bloom_build (one workgroup per filter segment, built in LDS)
-//   S:  k_bloom_filter (query + wave-ballot compaction S -> S')   [Bloom on]
-//       then the same two-level partition as R
-//   J:  k_plan_probe (vlsplit work items) -> k_probe (persistent; LDS chained
-//       hash table on the smaller side, stream the other side, LDS row staging)
+//   S:  bf_stage 2: k_hist + pass 1 of S, then k_filter_l1 (per first-level
+//       bucket, XCD-aware, filter slice L2-resident) places the survivors;
+//       bf_stage 1: k_bloom_prefilter (S -> S') then the two-level partition
+//   J:  k_plan_count/k_plan_fill (vlsplit work items) -> k_probe (persistent;
+//       LDS chained hash table on the smaller side, stream the other side)
 //
 // Reference counterparts (Chanho-LEE-SNU/BRIG-Join):
 //   k_hist / k_reduce_hist / scans   gpu_run_hist_for_both, gpu_make_simple_prefix,
@@ -15,7 +16,7 @@
 //   k_scatter                        gpu_make_sorted_table_for_both, gpu_build_table_second_pass
 //                                    (src/kernel_functions.cu:95-133, 210-281)
 //   k_bloom_build                    insertKernelPartitioned (src/bloom.cu:698-722)
-//   k_bloom_filter                   queryAndCompactKernelPartitioned (src/bloom.cu:725-773)
+//   k_bloom_prefilter, k_filter_l1   queryAndCompactKernelPartitioned (src/bloom.cu:725-773)
 //   k_probe                          gpu_probe_with_second_pass / gpu_fused_probe_transfer
 //                                    local part (src/kernel_functions.cu:294-457, 612-789)
 #include "bj_kernels.h"
@@ -149,20 +150,24 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const uint32_t* __restrict__ key
     for (int b = threadIdx.x; b < nbins; b += BLOCK) dst[b] = h[b];
 }
 
-// totals[b] = sum_g part[g][b]  (one thread per bin, coalesced across bins)
+// totals[b] += sum over a chunk of RCH groups of part[g][b]; grid (bins/256, G/RCH).
+// Loads are coalesced across bins; one atomic per (bin, chunk).  totals is zeroed first.
+constexpr int RCH = 32;
 __global__ void k_reduce_totals(const uint32_t* __restrict__ part, int G, int nbins,
                                 uint32_t* __restrict__ totals)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbins) return;
+    const int g0 = blockIdx.y * RCH, g1 = min(g0 + RCH, G);
     uint32_t s = 0;
-    for (int g = 0; g < G; g++) s += part[(size_t)g * nbins + b];
-    totals[b] = s;
+#pragma unroll 8
+    for (int g = g0; g < g1; g++) s += part[(size_t)g * nbins + b];
+    if (s) atomicAdd(&totals[b], s);
 }
 
-// l1part[b1 * G + g] = sum_{b2} part[g][(b1 << bits2) + b2]
+// l1h[g][b1] = sum_{b2} part[g][(b1 << bits2) + b2]  (row g of the first-level histogram)
 __global__ void k_reduce_l1(const uint32_t* __restrict__ part, int G, int bits, int bits2,
-                            uint32_t* __restrict__ l1part)
+                            uint32_t* __restrict__ l1h)
 {
     const int P1 = 1 << (bits - bits2);
     const int nbins = 1 << bits;
@@ -172,7 +177,50 @@ __global__ void k_reduce_l1(const uint32_t* __restrict__ part, int G, int bits, 
     const uint32_t* src = part + (size_t)g * nbins + ((size_t)b1 << bits2);
     uint32_t s = 0;
     for (int b2 = 0; b2 < (1 << bits2); b2++) s += src[b2];
-    l1part[(size_t)b1 * G + g] = s;
+    l1h[t] = s;
+}
+
+// First-level cursors of the pass-1 scatter: l1off[g][b1] = start of bucket b1
+// (sb_start[b1 << bits2]) + sum_{g' < g} l1h[g'][b1].  A column scan: each
+// workgroup owns 64 columns; 16 row groups per column are summed, scanned in
+// LDS, then rescanned row by row.  All loads/stores are coalesced across columns.
+__global__ __launch_bounds__(1024) void k_l1_offsets(const uint32_t* __restrict__ l1h, int G, int P1,
+                                                     int bits2, const uint64_t* __restrict__ sb_start,
+                                                     uint64_t* __restrict__ l1off)
+{
+    __shared__ uint64_t part_sum[16][64];
+    const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+    const int b1 = blockIdx.x * 64 + c;
+    const int rows = (G + 15) / 16;
+    const int g0 = min(r * rows, G), g1 = min(g0 + rows, G);
+    uint64_t s = 0;
+    if (b1 < P1)
+        for (int g = g0; g < g1; g++) s += l1h[(size_t)g * P1 + b1];
+    part_sum[r][c] = s;
+    __syncthreads();
+    if (r == 0 && b1 < P1) {
+        uint64_t acc = sb_start[(size_t)b1 << bits2];
+        for (int q = 0; q < 16; q++) {
+            const uint64_t v = part_sum[q][c];
+            part_sum[q][c] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    if (b1 < P1) {
+        uint64_t acc = part_sum[r][c];
+        for (int g = g0; g < g1; g++) {
+            l1off[(size_t)g * P1 + b1] = acc;
+            acc += l1h[(size_t)g * P1 + b1];
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
 // ============================================================== scan (u32 -> u64 exclusive)
@@ -242,36 +290,47 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t* __rest
 }
 
 // ============================================================== scatter
-// LDS-staged scatter.  Each tile of T tuples is counting-sorted by bucket in
-// LDS (rank from an LDS atomic, offsets from an LDS scan), then written out so
-// that consecutive lanes write consecutive addresses inside each bucket run.
-//   mode 0 (pass 1): workgroup g owns the same chunk as k_hist's group g;
-//                    bucket = h >> (32 - bits1); cursor[b] = l1off[b * G + g].
-//   mode 1 (pass 2): workgroup p owns first-level bucket p (its range is
-//                    sb_start[p << bits2] .. sb_start[(p+1) << bits2]);
-//                    bucket = (h >> (32 - bits)) & (P2 - 1);
-//                    cursor[b] = sb_start[(p << bits2) + b].
+// LDS-staged scatter with write-combining carries.
+//
+// Each tile of T tuples is counting-sorted by bucket in LDS (rank from an LDS
+// atomic, offsets from an LDS scan).  Bucket b's tuples go to consecutive global
+// positions starting at cur[b].  Only whole CS-tuple chunks (CS * 4 bytes) are
+// written; the unaligned tail of every bucket (< CS tuples) is carried in LDS
+// to the next tile, so every chunk of the output is written by one workgroup
+// within one tile.  Partial chunks occur only where two workgroups' ranges of
+// a bucket meet.  The next tile is loaded into registers while the current one
+// is written out.
+//   mode 0 (pass 1, p1 bits): workgroup g owns the same chunk as k_hist's group
+//       g; bucket = h >> (32 - p1); cur[b] = l1off[g * nbk + b].
+//   mode 1 (pass 2, bits - p1 bits): workgroup (b1, w) owns the pass-1 output of
+//       bucket b1 written by pass-1 groups [w*G/W2, (w+1)*G/W2); bucket =
+//       (h >> (32 - bits)) & (2^(bits-p1) - 1); cur[b] = l2cur[w][(b1 << (bits-p1)) + b].
 // Writes are bounds-checked against out_cap and flag err on violation.
 template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_scatter(
     const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ ival,
     uint32_t* __restrict__ okey, uint32_t* __restrict__ oval, uint64_t out_cap,
-    uint64_t n_host, const uint64_t* __restrict__ n_dev, int G, int mode, int bits, int bits2,
-    const uint64_t* __restrict__ l1off, const uint64_t* __restrict__ sb_start,
-    uint64_t* __restrict__ err)
+    uint64_t n_host, const uint64_t* __restrict__ n_dev, int G, int mode, int bits, int p1, int W2,
+    int cs, const uint64_t* __restrict__ l1off, const uint64_t* __restrict__ sb_start,
+    const uint64_t* __restrict__ l2cur, uint64_t* __restrict__ err)
 {
     constexpr int T = BLOCK * ITEMS;
-    const int nbk = (mode == 0) ? (1 << (bits - bits2)) : (1 << bits2);
-    const int shift = (mode == 0) ? (32 - (bits - bits2)) : (32 - bits);
+    const int b2 = bits - p1;
+    const int nbk = (mode == 0) ? (1 << p1) : (1 << b2);
+    const int shift = (mode == 0) ? (32 - p1) : (32 - bits);
     const uint32_t bmask = (uint32_t)nbk - 1u;
+    const uint64_t amask = ~(uint64_t)(cs - 1);
 
     extern __shared__ uint64_t lds64[];
     uint64_t* cur = lds64;                                 // [nbk]
-    uint32_t* skey = (uint32_t*)(cur + nbk);               // [T]
+    uint32_t* ck = (uint32_t*)(cur + nbk);                 // [nbk * cs] carried keys
+    uint32_t* cv = ck + nbk * cs;                          // [nbk * cs] carried values
+    uint32_t* skey = cv + nbk * cs;                        // [T]
     uint32_t* sval = skey + T;                             // [T]
     uint32_t* cnt = sval + T;                              // [nbk]
     uint32_t* off = cnt + nbk;                             // [nbk]
-    uint32_t* wtmp = off + nbk;                            // [BLOCK/64 + 1]
+    uint32_t* ccnt = off + nbk;                            // [nbk]
+    uint32_t* wtmp = ccnt + nbk;                           // [BLOCK/64 + 1]
 
     uint64_t lo, hi;
     if (mode == 0) {
@@ -279,27 +338,37 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
         const uint64_t chunk = (n + (uint64_t)G - 1) / (uint64_t)G;
         lo = min((uint64_t)blockIdx.x * chunk, n);
         hi = min(lo + chunk, n);
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) cur[b] = l1off[(size_t)b * G + blockIdx.x];
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) cur[b] = l1off[(size_t)blockIdx.x * nbk + b];
     } else {
-        const uint64_t p = blockIdx.x;
-        lo = sb_start[p << bits2];
-        hi = sb_start[(p + 1) << bits2];
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) cur[b] = sb_start[(p << bits2) + b];
+        const int P1 = 1 << p1;
+        const int b1 = blockIdx.x / W2, w = blockIdx.x % W2;
+        const int g0 = (int)(((int64_t)w * G) / W2), g1 = (int)(((int64_t)(w + 1) * G) / W2);
+        lo = (g0 < G) ? l1off[(size_t)g0 * P1 + b1] : sb_start[(uint64_t)(b1 + 1) << b2];
+        hi = (g1 < G) ? l1off[(size_t)g1 * P1 + b1] : sb_start[(uint64_t)(b1 + 1) << b2];
+        const uint64_t nbins = 1ull << bits;
+        for (int b = threadIdx.x; b < nbk; b += BLOCK)
+            cur[b] = l2cur[(size_t)w * nbins + ((uint64_t)b1 << b2) + b];
     }
+    for (int b = threadIdx.x; b < nbk; b += BLOCK) ccnt[b] = 0u;
+    // cur/ccnt are read by other threads; an empty range skips the tile loop's barriers
+    __syncthreads();
+
+    uint32_t k[ITEMS], v[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint64_t i = lo + (uint64_t)it * BLOCK + threadIdx.x;
+        if (i < hi) {
+            k[it] = __builtin_nontemporal_load(&ikey[i]);
+            v[it] = __builtin_nontemporal_load(&ival[i]);
+        }
+    }
+    bool bad = false;
 
     for (uint64_t tile = lo; tile < hi; tile += T) {
         const int tn = (int)min((uint64_t)T, hi - tile);
         for (int b = threadIdx.x; b < nbk; b += BLOCK) cnt[b] = 0u;
         __syncthreads();
-        uint32_t k[ITEMS], v[ITEMS], bk[ITEMS], rk[ITEMS];
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const int s = it * BLOCK + threadIdx.x;
-            if (s < tn) {
-                k[it] = ikey[tile + s];
-                v[it] = ival[tile + s];
-            }
-        }
+        uint32_t bk[ITEMS], rk[ITEMS];
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
             const int s = it * BLOCK + threadIdx.x;
@@ -320,69 +389,155 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
                 sval[p] = v[it];
             }
         }
-        __syncthreads();
-        bool bad = false;
+        // prefetch the next tile while this one is written out
+        const uint64_t next = tile + T;
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
-            const int s = it * BLOCK + threadIdx.x;
-            if (s < tn) {
-                const uint32_t kk = skey[s];
-                const uint32_t b = (radix_hash(kk) >> shift) & bmask;
-                const uint64_t dst = cur[b] + (uint64_t)(s - (int)off[b]);
-                if (dst < out_cap) {
-                    okey[dst] = kk;
-                    oval[dst] = sval[s];
-                } else {
-                    bad = true;
+            const uint64_t i = next + (uint64_t)it * BLOCK + threadIdx.x;
+            if (i < hi) {
+                k[it] = __builtin_nontemporal_load(&ikey[i]);
+                v[it] = __builtin_nontemporal_load(&ival[i]);
+            }
+        }
+        __syncthreads();
+        // phase A: carried tuples below the write end go out
+        for (int idx = threadIdx.x; idx < nbk * cs; idx += BLOCK) {
+            const int b = idx / cs, i = idx - b * cs;
+            const uint32_t cc = ccnt[b];
+            if ((uint32_t)i < cc) {
+                const uint64_t c0 = cur[b];
+                const uint64_t e = c0 + cc + cnt[b];
+                const uint64_t wend = max(e & amask, c0);
+                const uint64_t g = c0 + (uint64_t)i;
+                if (g < wend) {
+                    if (g < out_cap) {
+                        okey[g] = ck[idx];
+                        oval[g] = cv[idx];
+                    } else {
+                        bad = true;
+                    }
                 }
             }
         }
-        if (bad) atomicOr((unsigned long long*)err, 1ull);
         __syncthreads();
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) cur[b] += cnt[b];
+        // phase B: the tile's tuples go out, or into the carry below CS
+        for (int s = threadIdx.x; s < tn; s += BLOCK) {
+            const uint32_t kk = skey[s];
+            const uint32_t b = (radix_hash(kk) >> shift) & bmask;
+            const uint64_t c0 = cur[b];
+            const uint32_t cc = ccnt[b];
+            const uint64_t e = c0 + cc + cnt[b];
+            const uint64_t wend = max(e & amask, c0);
+            const uint64_t g = c0 + cc + (uint64_t)(s - (int)off[b]);
+            if (g < wend) {
+                if (g < out_cap) {
+                    okey[g] = kk;
+                    oval[g] = sval[s];
+                } else {
+                    bad = true;
+                }
+            } else {
+                const uint32_t slot = (uint32_t)(g - wend);
+                ck[b * cs + slot] = kk;
+                cv[b * cs + slot] = sval[s];
+            }
+        }
         __syncthreads();
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
+            const uint64_t c0 = cur[b];
+            const uint64_t e = c0 + ccnt[b] + cnt[b];
+            const uint64_t wend = max(e & amask, c0);
+            ccnt[b] = (uint32_t)(e - wend);
+            cur[b] = wend;
+        }
+        __syncthreads();
+    }
+    // the remaining carries (partial chunks at the range ends)
+    for (int idx = threadIdx.x; idx < nbk * cs; idx += BLOCK) {
+        const int b = idx / cs, i = idx - b * cs;
+        if ((uint32_t)i < ccnt[b]) {
+            const uint64_t g = cur[b] + (uint64_t)i;
+            if (g < out_cap) {
+                okey[g] = ck[idx];
+                oval[g] = cv[idx];
+            } else {
+                bad = true;
+            }
+        }
+    }
+    if (bad) atomicOr((unsigned long long*)err, 1ull);
+}
+
+// Pass-2 cursors.  ps[w][sb] = sum of part[g][sb] over pass-1 groups g in
+// group w (w < W2); then an exclusive scan over w per sub-bucket, offset by
+// sb_start[sb]: l2cur[w][sb] = where pass-2 workgroup (b1, w) writes its first
+// tuple of sub-bucket sb.  Coalesced across sub-buckets.
+__global__ void k_l2_offsets(const uint32_t* __restrict__ part, int G, int nbins, int W2,
+                             const uint64_t* __restrict__ sb_start, uint64_t* __restrict__ l2cur)
+{
+    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= nbins) return;
+    uint64_t acc = sb_start[sb];
+    for (int w = 0; w < W2; w++) {
+        l2cur[(size_t)w * nbins + sb] = acc;
+        const int g0 = (int)(((int64_t)w * G) / W2), g1 = (int)(((int64_t)(w + 1) * G) / W2);
+        uint32_t s = 0;
+#pragma unroll 8
+        for (int g = g0; g < g1; g++) s += part[(size_t)g * nbins + sb];
+        acc += s;
     }
 }
 
 // ============================================================== Bloom build
-// One workgroup per filter segment: zero the segment in LDS, OR in the k bits
-// of every R key of the segment's sub-buckets, write the segment once.
+// One workgroup per filter segment: zero the segment in LDS, OR in the block
+// of every R key of the segment's sub-buckets (one 64-bit LDS atomic per key),
+// write the segment once.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_bloom_build(const uint32_t* __restrict__ rkey,
                                                        const uint64_t* __restrict__ r_start, int bits,
-                                                       BloomGeom g, uint32_t* __restrict__ filter)
+                                                       BloomGeom g, uint64_t* __restrict__ filter)
 {
-    extern __shared__ uint32_t seg[];
+    extern __shared__ unsigned long long seg[];
     const uint32_t wps = g.wps;
-    for (uint32_t i = threadIdx.x; i < wps; i += BLOCK) seg[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < wps; i += BLOCK) seg[i] = 0ull;
     __syncthreads();
     const uint64_t s = blockIdx.x;
     const int d = bits - g.seg_bits;
     const uint64_t lo = r_start[s << d], hi = r_start[(s + 1) << d];
     const uint64_t segbase = s * (uint64_t)wps;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += BLOCK) {
-        BloomAddr a;
-        bloom_addr(rkey[i], g, a);
-        const uint32_t wb = (uint32_t)(a.block_word - segbase);
+    constexpr int U = 8;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += (uint64_t)U * BLOCK) {
+        uint32_t kk[U];
 #pragma unroll
-        for (int w = 0; w < 8; w++)
-            if (a.mask[w]) atomicOr(&seg[wb + w], a.mask[w]);
+        for (int u = 0; u < U; u++) {
+            const uint64_t j = i + (uint64_t)u * BLOCK;
+            kk[u] = (j < hi) ? __builtin_nontemporal_load(&rkey[j]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (i + (uint64_t)u * BLOCK < hi) {
+                const uint32_t w = (uint32_t)(bloom_block(kk[u], g) - segbase);
+                atomicOr(&seg[w], (unsigned long long)bloom_mask(kk[u], g.k));
+            }
+        }
     }
     __syncthreads();
-    uint4* dst = (uint4*)(filter + segbase);
-    const uint4* src = (const uint4*)seg;
-    for (uint32_t i = threadIdx.x; i < wps / 4; i += BLOCK) dst[i] = src[i];
+    uint64_t* dst = filter + segbase;
+    for (uint32_t i = threadIdx.x; i < wps; i += BLOCK) dst[i] = seg[i];
 }
 
-// ============================================================== Bloom filter S -> S'
+// ============================================================== Bloom filter S -> S' (before partitioning)
+// The reference's order (queryAndCompactKernelPartitioned): S is filtered
+// before it is partitioned.  Used when the filter fits the MALL, since each key
+// probes a random 256-bit block.
 // Query each S key's 256-bit block (two 16-byte loads), keep the survivors.
 // Compaction: wave ballot + mbcnt inside each (item, wave) group, an LDS scan
 // over the groups, one global atomic per tile.  s_val is read only for
 // survivors.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_bloom_filter(
+__global__ __launch_bounds__(BLOCK) void k_bloom_prefilter(
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval, uint64_t n,
-    const uint32_t* __restrict__ filter, BloomGeom g, uint32_t* __restrict__ okey,
+    const uint64_t* __restrict__ filter, BloomGeom g, uint32_t* __restrict__ okey,
     uint32_t* __restrict__ oval, uint64_t out_cap, uint64_t* __restrict__ count,
     uint64_t* __restrict__ err)
 {
@@ -401,21 +556,15 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_filter(
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
             const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
-            k[it] = (i < n) ? skey[i] : 0u;
+            k[it] = (i < n) ? __builtin_nontemporal_load(&skey[i]) : 0u;
         }
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
             const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
             bool ok = false;
             if (i < n) {
-                BloomAddr a;
-                bloom_addr(k[it], g, a);
-                const uint4* blk = (const uint4*)(filter + a.block_word);
-                const uint4 lo4 = blk[0], hi4 = blk[1];
-                ok = ((lo4.x & a.mask[0]) == a.mask[0]) && ((lo4.y & a.mask[1]) == a.mask[1]) &&
-                     ((lo4.z & a.mask[2]) == a.mask[2]) && ((lo4.w & a.mask[3]) == a.mask[3]) &&
-                     ((hi4.x & a.mask[4]) == a.mask[4]) && ((hi4.y & a.mask[5]) == a.mask[5]) &&
-                     ((hi4.z & a.mask[6]) == a.mask[6]) && ((hi4.w & a.mask[7]) == a.mask[7]);
+                const uint64_t m = bloom_mask(k[it], g.k);
+                ok = (filter[bloom_block(k[it], g)] & m) == m;
             }
             pass[it] = ok;
         }
@@ -459,126 +608,235 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_filter(
     }
 }
 
+// ============================================================== Bloom filter after pass 1
+// S has been scattered by its first-pass bucket.  The filter slice of
+// first-pass bucket p is contiguous (segments are the top radix bits), 1/P1 of
+// the filter.  Workgroups are dealt to XCDs round-robin (workgroup i runs on
+// XCD i % 8), so XCD x walks buckets x, x+8, ... with all its workgroups on the
+// same bucket at the same time: that bucket's filter slice stays in the XCD's
+// L2 while its S tuples stream past (loaded non-temporally).
+// Survivors are compacted into an LDS buffer (wave ballot + one LDS atomic per
+// wave) and counted per sub-bucket.  A flush reserves each sub-bucket's rows
+// with one global atomic and writes them into the sub-bucket's region of the S
+// layout (sub-bucket sb gets positions sb_start[sb] + [0, s_fill[sb])), so no
+// second pass over S is needed.  Values are read only for survivors.
+template <int BLOCK, int ITEMS, int BUF>
+__global__ __launch_bounds__(BLOCK) void k_filter_l1(
+    const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ ival,
+    const uint64_t* __restrict__ sb_start, int bits, int bits2, const uint64_t* __restrict__ filter,
+    BloomGeom g, uint32_t* __restrict__ okey, uint32_t* __restrict__ oval, uint64_t out_cap,
+    uint32_t* __restrict__ s_fill, uint64_t* __restrict__ count, uint64_t* __restrict__ err)
+{
+    constexpr int T = BLOCK * ITEMS;
+    constexpr int NXCD = 8;
+    static_assert(BUF >= T, "the buffer holds at least one tile");
+    const int P1 = 1 << (bits - bits2), P2 = 1 << bits2;
+    const int shift = 32 - bits;
+    const uint32_t bmask = (uint32_t)P2 - 1u;
+    extern __shared__ uint64_t lds64[];
+    uint64_t* base = lds64;                       // [P2] reserved global start per sub-bucket
+    uint32_t* bkey = (uint32_t*)(base + P2);      // [BUF] buffered survivors
+    uint32_t* bval = bkey + BUF;                  // [BUF]
+    uint32_t* cnt = bval + BUF;                   // [P2]
+    uint32_t* wcur = cnt + P2;                    // [P2]
+    __shared__ uint32_t s_buf;
+
+    const int xcd = blockIdx.x % NXCD;
+    const int W = gridDim.x / NXCD;
+    const int j = blockIdx.x / NXCD;
+    if (j >= W) return;  // grid is a multiple of NXCD; uniform per workgroup
+    const uint32_t lane = lane_id();
+    bool bad = false;
+    uint64_t surv_total = 0;
+
+    for (int b = threadIdx.x; b < P2; b += BLOCK) cnt[b] = 0u;
+    if (threadIdx.x == 0) s_buf = 0u;
+    __syncthreads();
+
+    // write the buffered survivors of first-pass bucket p; every thread calls it
+    auto flush = [&](int p) {
+        const uint32_t n = s_buf;
+        for (int b = threadIdx.x; b < P2; b += BLOCK) {
+            const uint64_t sb = ((uint64_t)p << bits2) + b;
+            const uint32_t c = cnt[b];
+            base[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
+            wcur[b] = 0u;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+            const uint32_t kk = bkey[i];
+            const uint32_t b = (radix_hash(kk) >> shift) & bmask;
+            const uint64_t d = base[b] + atomicAdd(&wcur[b], 1u);
+            if (d < out_cap) {
+                okey[d] = kk;
+                oval[d] = bval[i];
+            } else {
+                bad = true;
+            }
+        }
+        surv_total += n;
+        __syncthreads();
+        for (int b = threadIdx.x; b < P2; b += BLOCK) cnt[b] = 0u;
+        if (threadIdx.x == 0) s_buf = 0u;
+        __syncthreads();
+    };
+
+    for (int p = xcd; p < P1; p += NXCD) {
+        const uint64_t lo = sb_start[(uint64_t)p << bits2];
+        const uint64_t hi = sb_start[(uint64_t)(p + 1) << bits2];
+        const uint64_t chunk = (hi - lo + W - 1) / W;
+        const uint64_t clo = min(lo + (uint64_t)j * chunk, hi);
+        const uint64_t chi = min(clo + chunk, hi);
+        for (uint64_t tile = clo; tile < chi; tile += T) {
+            if (s_buf + (uint32_t)T > (uint32_t)BUF) flush(p);  // s_buf is stable here
+            const int tn = (int)min((uint64_t)T, chi - tile);
+            uint32_t k[ITEMS];
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++) {
+                const int s = it * BLOCK + threadIdx.x;
+                k[it] = (s < tn) ? __builtin_nontemporal_load(&ikey[tile + s]) : 0u;
+            }
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++) {
+                const int s = it * BLOCK + threadIdx.x;
+                bool ok = false;
+                if (s < tn) {
+                    const uint64_t m = bloom_mask(k[it], g.k);
+                    ok = (filter[bloom_block(k[it], g)] & m) == m;
+                }
+                const uint64_t bal = __ballot(ok);
+                uint32_t wb = 0;
+                if (lane == 0 && bal) wb = atomicAdd(&s_buf, (uint32_t)__popcll(bal));
+                wb = __shfl(wb, 0, WAVE);
+                if (ok) {
+                    const uint32_t slot = wb + lanes_below(bal);
+                    bkey[slot] = k[it];
+                    bval[slot] = __builtin_nontemporal_load(&ival[tile + s]);
+                    atomicAdd(&cnt[(radix_hash(k[it]) >> shift) & bmask], 1u);
+                }
+            }
+            __syncthreads();
+        }
+        flush(p);
+    }
+    if (threadIdx.x == 0 && surv_total)
+        atomicAdd((unsigned long long*)count, (unsigned long long)surv_total);
+    if (bad) atomicOr((unsigned long long*)err, 8ull);
+}
+
 // ============================================================== probe planning
-// Sub-buckets whose smaller side fits one LDS tile and whose larger side fits
-// one slice are "regular" and are processed as one work item, indexed by the
-// sub-bucket itself.  Larger ones are split (vlsplit): the build side into
-// ceil(nb / tile) chunks, the probe side into ceil(np / slice) slices, one
-// explicit work item per (chunk, slice).  If the item buffer would overflow,
-// a sub-bucket falls back to one slice per chunk (always fits: the buffer holds
-// at least sum(ceil(nb/tile)) items).
-__global__ void k_plan_probe(const uint64_t* __restrict__ r_start, const uint64_t* __restrict__ s_start,
-                             int nbins, uint32_t tile, uint32_t slice, HeavyItem* __restrict__ items,
-                             uint64_t item_cap, uint64_t* __restrict__ n_items,
-                             uint64_t* __restrict__ err)
+// A sub-bucket whose smaller side fits one LDS tile and whose larger side fits
+// one slice is "regular": one work item, indexed by the sub-bucket itself.
+// Larger ones are split (vlsplit): the build side into ceil(nb / tile) chunks,
+// the probe side into min(ceil(np / slice), max_slices) slices, one explicit
+// work item per (chunk, slice).  Count, scan, fill: no contention, and the
+// item buffer bound max_slices * (sum ceil(nb / tile)) always holds.
+__device__ __forceinline__ void sb_sizes(const uint64_t* r_start, const uint64_t* s_start,
+                                         const uint32_t* s_cnt, int sb, uint64_t& r0, uint64_t& nr,
+                                         uint64_t& s0, uint64_t& ns)
+{
+    r0 = r_start[sb];
+    nr = r_start[sb + 1] - r0;
+    s0 = s_start[sb];
+    ns = s_cnt ? (uint64_t)s_cnt[sb] : (s_start[sb + 1] - s0);
+}
+
+__device__ __forceinline__ uint32_t plan_items(uint64_t nr, uint64_t ns, uint32_t tile, uint32_t slice,
+                                               uint32_t max_slices, uint32_t& chunks, uint32_t& slices)
+{
+    chunks = slices = 0;
+    if (nr == 0 || ns == 0) return 0;
+    const uint64_t nb = min(nr, ns), np = max(nr, ns);
+    if (nb <= tile && np <= slice) return 0;
+    chunks = (uint32_t)((nb + tile - 1) / tile);
+    slices = (uint32_t)min<uint64_t>((np + slice - 1) / slice, max_slices);
+    return chunks * slices;
+}
+
+__global__ void k_plan_count(const uint64_t* __restrict__ r_start, const uint64_t* __restrict__ s_start,
+                             const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile,
+                             uint32_t slice, uint32_t max_slices, uint32_t* __restrict__ n_items)
 {
     const int sb = blockIdx.x * blockDim.x + threadIdx.x;
     if (sb >= nbins) return;
-    const uint64_t nr = r_start[sb + 1] - r_start[sb];
-    const uint64_t ns = s_start[sb + 1] - s_start[sb];
-    if (nr == 0 || ns == 0) return;
-    const uint64_t nb = min(nr, ns), np = max(nr, ns);
-    if (nb <= tile && np <= slice) return;
-    const uint64_t chunks = (nb + tile - 1) / tile;
-    uint64_t slices = (np + slice - 1) / slice;
-    unsigned long long cur = atomicAdd((unsigned long long*)n_items, 0ull);
-    unsigned long long take;
-    for (;;) {
-        take = chunks * slices;
-        if (cur + take > item_cap) {
-            slices = 1;
-            take = chunks;
-        }
-        if (cur + take > item_cap) {
-            atomicOr((unsigned long long*)err, 4ull);
-            return;
-        }
-        const unsigned long long prev =
-            atomicCAS((unsigned long long*)n_items, cur, cur + take);
-        if (prev == cur) break;
-        cur = prev;
-        slices = (np + slice - 1) / slice;
+    uint64_t r0, nr, s0, ns;
+    sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
+    uint32_t c, sl;
+    n_items[sb] = plan_items(nr, ns, tile, slice, max_slices, c, sl);
+}
+
+__global__ void k_plan_fill(const uint64_t* __restrict__ r_start, const uint64_t* __restrict__ s_start,
+                            const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile, uint32_t slice,
+                            uint32_t max_slices, const uint64_t* __restrict__ item_start,
+                            uint64_t item_cap, HeavyItem* __restrict__ items, uint64_t* __restrict__ err)
+{
+    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= nbins) return;
+    uint64_t r0, nr, s0, ns;
+    sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
+    uint32_t chunks, slices;
+    const uint32_t n = plan_items(nr, ns, tile, slice, max_slices, chunks, slices);
+    const uint64_t base = item_start[sb];
+    if (base + n > item_cap) {
+        atomicOr((unsigned long long*)err, 4ull);
+        return;
     }
-    for (uint64_t c = 0; c < chunks; c++)
-        for (uint64_t s = 0; s < slices; s++) {
+    for (uint32_t c = 0; c < chunks; c++)
+        for (uint32_t s = 0; s < slices; s++) {
             HeavyItem it;
             it.sb = (uint32_t)sb;
-            it.chunk = (uint32_t)c;
-            it.slice = (uint32_t)s;
-            it.nslices = (uint32_t)slices;
-            items[cur + c * slices + s] = it;
+            it.chunk = c;
+            it.slice = s;
+            it.nslices = slices;
+            items[base + (uint64_t)c * slices + s] = it;
         }
 }
 
 // ============================================================== build + probe
-// Persistent workgroups pull work items from a global counter: first the
-// vlsplit items (largest first), then one item per regular sub-bucket.
-// Per item: the smaller side's chunk is loaded into an LDS chained hash table
-// (head[] of index+1 per bucket, nxt[] links), then the other side's slice is
-// streamed through it.  Matches are staged in LDS as (key, build value, probe
-// row) and flushed with one global atomic per flush; the probe side's value is
-// read only for matching rows, at flush time.
+// Persistent workgroups pull work items from a global counter (the next item
+// is fetched while the current one runs): first the vlsplit items (the
+// largest), then one item per regular sub-bucket.
+// Per item the smaller side's chunk goes into an LDS chained hash table
+// (head[] holds index+1 per bucket, nxt[] links; keys only, 6 bytes per build
+// tuple, so two workgroups fit a CU).  The other side's slice then streams
+// through it twice, in rounds of BLOCK*PITEMS keys:
+//   pass 1 counts the item's matches (no barriers); one global atomic per item
+//          reserves its rows;
+//   pass 2 recounts per round, places each lane's rows after a block scan and
+//          writes them.  Its key re-reads are cache hits (the slice was just read).
+// Both values of a row are read only for matches: the build side's from its
+// (cache-resident) chunk, the probe side's from its slice.
 template <int BLOCK, int CAP, int PITEMS>
 __global__ __launch_bounds__(BLOCK) void k_probe(
     const uint32_t* __restrict__ rk, const uint32_t* __restrict__ rv,
     const uint64_t* __restrict__ r_start, const uint32_t* __restrict__ sk,
-    const uint32_t* __restrict__ sv, const uint64_t* __restrict__ s_start, int bits,
-    uint32_t slice, const HeavyItem* __restrict__ items, const uint64_t* __restrict__ n_items_p,
+    const uint32_t* __restrict__ sv, const uint64_t* __restrict__ s_start,
+    const uint32_t* __restrict__ s_cnt, int bits, uint32_t slice,
+    const HeavyItem* __restrict__ items, const uint64_t* __restrict__ n_items_p, uint64_t item_cap,
     uint64_t* __restrict__ work_ctr, uint64_t* __restrict__ out_count, uint64_t out_cap,
     uint32_t* __restrict__ o_key, uint32_t* __restrict__ o_rval, uint32_t* __restrict__ o_sval)
 {
     constexpr int NBMAX = CAP / 2;
-    constexpr int OB = 2 * BLOCK;
+    constexpr int ROUND = BLOCK * PITEMS;
+    constexpr int BU = 4;  // build loads in flight per thread
     __shared__ uint32_t hkey[CAP];
-    __shared__ uint32_t hval[CAP];
     __shared__ uint16_t nxt[CAP];
     __shared__ uint32_t head[NBMAX];
-    __shared__ uint32_t ob_key[OB];
-    __shared__ uint32_t ob_bval[OB];
-    __shared__ uint32_t ob_pidx[OB];
-    __shared__ uint32_t s_ocnt;
-    __shared__ uint64_t s_item;
+    __shared__ uint32_t wtmp[BLOCK / WAVE + 1];
+    __shared__ uint64_t s_item, s_next;
     __shared__ unsigned long long s_base;
 
     const int nbins = 1 << bits;
-    const uint64_t n_heavy = *n_items_p;
+    const uint64_t n_heavy = min(*n_items_p, item_cap);
     const uint64_t n_work = n_heavy + (uint64_t)nbins;
-    if (threadIdx.x == 0) s_ocnt = 0u;
+
+    if (threadIdx.x == 0) s_item = atomicAdd((unsigned long long*)work_ctr, 1ull);
+    __syncthreads();
+    uint64_t item = s_item;
     __syncthreads();
 
-    // Flush the staged rows; every thread must call it (contains barriers).
-    auto flush = [&](uint32_t staged) {
-        const uint32_t nrow = min(staged, (uint32_t)OB);
-        if (threadIdx.x == 0)
-            s_base = nrow ? atomicAdd((unsigned long long*)out_count, (unsigned long long)nrow) : 0ull;
-        __syncthreads();
-        const uint64_t base = s_base;
-        for (uint32_t i = threadIdx.x; i < nrow; i += BLOCK) {
-            const uint64_t gi = base + i;
-            if (gi < out_cap) {
-                const uint32_t pid = ob_pidx[i];
-                const uint32_t prow = pid & 0x7FFFFFFFu;
-                o_key[gi] = ob_key[i];
-                if (pid & 0x80000000u) {  // build side S, probe side R
-                    o_rval[gi] = rv[prow];
-                    o_sval[gi] = ob_bval[i];
-                } else {                  // build side R, probe side S
-                    o_rval[gi] = ob_bval[i];
-                    o_sval[gi] = sv[prow];
-                }
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) s_ocnt = 0u;
-        __syncthreads();
-    };
-
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd((unsigned long long*)work_ctr, 1ull);
-        __syncthreads();
-        const uint64_t item = s_item;
-        __syncthreads();
-        if (item >= n_work) break;
+    while (item < n_work) {
+        if (threadIdx.x == 0) s_next = atomicAdd((unsigned long long*)work_ctr, 1ull);
 
         uint32_t sb, chunk = 0, sl = 0, nsl = 1;
         const bool heavy = item < n_heavy;
@@ -588,89 +846,144 @@ __global__ __launch_bounds__(BLOCK) void k_probe(
         } else {
             sb = (uint32_t)(item - n_heavy);
         }
-        const uint64_t r0 = r_start[sb], nr = r_start[sb + 1] - r0;
-        const uint64_t s0 = s_start[sb], ns = s_start[sb + 1] - s0;
-        if (nr == 0 || ns == 0) continue;
+        uint64_t r0, nr, s0, ns;
+        sb_sizes(r_start, s_start, s_cnt, (int)sb, r0, nr, s0, ns);
         const bool build_r = nr <= ns;
         const uint64_t nb = build_r ? nr : ns, np = build_r ? ns : nr;
-        if (!heavy && (nb > (uint64_t)CAP || np > (uint64_t)slice)) continue;  // on the vlsplit list
+        // empty, or a regular index whose sub-bucket is on the vlsplit list: nothing to do
+        const bool skip = nr == 0 || ns == 0 || (!heavy && (nb > (uint64_t)CAP || np > (uint64_t)slice));
+        if (!skip) {
+            const uint64_t b_lo = (uint64_t)chunk * CAP;
+            const int bn = (int)min((uint64_t)CAP, nb - b_lo);
+            const uint64_t per = (np + nsl - 1) / nsl;
+            const uint64_t p_lo = min((uint64_t)sl * per, np);
+            const uint64_t pn = min(per, np - p_lo);
+            const uint32_t* bkp = (build_r ? rk : sk) + (build_r ? r0 : s0) + b_lo;
+            const uint32_t* bvp = (build_r ? rv : sv) + (build_r ? r0 : s0) + b_lo;
+            const uint64_t prow0 = (build_r ? s0 : r0) + p_lo;
+            const uint32_t* pkp = (build_r ? sk : rk) + prow0;
+            const uint32_t* pvp = (build_r ? sv : rv) + prow0;
 
-        const uint64_t b_lo = (uint64_t)chunk * CAP;
-        const int bn = (int)min((uint64_t)CAP, nb - b_lo);
-        const uint64_t per = (np + nsl - 1) / nsl;
-        const uint64_t p_lo = min((uint64_t)sl * per, np);
-        const uint64_t pn = min(per, np - p_lo);
-        const uint32_t* bkp = (build_r ? rk : sk) + (build_r ? r0 : s0) + b_lo;
-        const uint32_t* bvp = (build_r ? rv : sv) + (build_r ? r0 : s0) + b_lo;
-        const uint64_t prow0 = (build_r ? s0 : r0) + p_lo;
-        const uint32_t* pkp = (build_r ? sk : rk) + prow0;
-        const uint32_t pflag = build_r ? 0u : 0x80000000u;
-
-        // table geometry: 2^tb buckets, tb = ceil(log2 bn) - 1 (load 1..2)
-        int tb = 0;
-        while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= NBMAX) tb++;
-        const int nbk = 1 << tb;
-        const int tshift = 32 - tb;
-        for (int i = threadIdx.x; i < nbk; i += BLOCK) head[i] = 0u;
-        __syncthreads();
-        for (int i = threadIdx.x; i < bn; i += BLOCK) {
-            const uint32_t k = bkp[i];
-            hkey[i] = k;
-            hval[i] = bvp[i];
-            const uint32_t t = tb ? ((radix_hash(k) << bits) >> tshift) : 0u;
-            nxt[i] = (uint16_t)atomicExch(&head[t], (uint32_t)(i + 1));
-        }
-        __syncthreads();
-
-        for (uint64_t base = 0; base < pn; base += (uint64_t)BLOCK * PITEMS) {
-            uint32_t kk[PITEMS];
+            // table geometry: 2^tb buckets, about 1..2 build tuples per bucket
+            int tb = 0;
+            while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= NBMAX) tb++;
+            const int nbk = 1 << tb;
+            const int tshift = 32 - tb;
+            for (int i = threadIdx.x; i < nbk; i += BLOCK) head[i] = 0u;
+            __syncthreads();
+            for (int i0 = threadIdx.x; i0 < bn; i0 += BU * BLOCK) {
+                uint32_t kb[BU];
 #pragma unroll
-            for (int it = 0; it < PITEMS; it++) {
-                const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
-                kk[it] = (idx < pn) ? pkp[idx] : 0u;
+                for (int u = 0; u < BU; u++) {
+                    const int i = i0 + u * BLOCK;
+                    kb[u] = (i < bn) ? bkp[i] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < BU; u++) {
+                    const int i = i0 + u * BLOCK;
+                    if (i < bn) {
+                        hkey[i] = kb[u];
+                        const uint32_t t = tb ? ((radix_hash(kb[u]) << bits) >> tshift) : 0u;
+                        nxt[i] = (uint16_t)atomicExch(&head[t], (uint32_t)(i + 1));
+                    }
+                }
             }
+            __syncthreads();
+
+            // pass 1: count the item's matches
+            uint32_t mine = 0;
+            {
+                uint32_t nk[PITEMS];
 #pragma unroll
-            for (int it = 0; it < PITEMS; it++) {
-                const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
-                if (idx < pn) {
-                    const uint32_t k = kk[it];
-                    const uint32_t t = tb ? ((radix_hash(k) << bits) >> tshift) : 0u;
-                    uint32_t c = head[t];
-                    while (c) {
-                        const uint32_t j = c - 1u;
-                        if (hkey[j] == k) {
-                            const uint32_t pid = (uint32_t)(prow0 + idx) | pflag;
-                            const uint32_t slot = atomicAdd(&s_ocnt, 1u);
-                            if (slot < (uint32_t)OB) {
-                                ob_key[slot] = k;
-                                ob_bval[slot] = hval[j];
-                                ob_pidx[slot] = pid;
-                            } else {
-                                // staging full (duplicate keys): write this row directly
-                                const unsigned long long gi =
-                                    atomicAdd((unsigned long long*)out_count, 1ull);
-                                if (gi < out_cap) {
-                                    const uint32_t prow = pid & 0x7FFFFFFFu;
-                                    o_key[gi] = k;
-                                    if (build_r) { o_rval[gi] = hval[j]; o_sval[gi] = sv[prow]; }
-                                    else         { o_rval[gi] = rv[prow]; o_sval[gi] = hval[j]; }
+                for (int it = 0; it < PITEMS; it++) {
+                    const uint64_t idx = (uint64_t)it * BLOCK + threadIdx.x;
+                    nk[it] = (idx < pn) ? pkp[idx] : 0u;
+                }
+                for (uint64_t base = 0; base < pn; base += ROUND) {
+                    uint32_t kk[PITEMS];
+#pragma unroll
+                    for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
+#pragma unroll
+                    for (int it = 0; it < PITEMS; it++) {
+                        const uint64_t idx = base + ROUND + (uint64_t)it * BLOCK + threadIdx.x;
+                        if (idx < pn) nk[it] = pkp[idx];
+                    }
+#pragma unroll
+                    for (int it = 0; it < PITEMS; it++) {
+                        const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
+                        if (idx < pn) {
+                            const uint32_t k = kk[it];
+                            for (uint32_t e = head[tb ? ((radix_hash(k) << bits) >> tshift) : 0u]; e;
+                                 e = nxt[e - 1])
+                                mine += (hkey[e - 1] == k);
+                        }
+                    }
+                }
+            }
+            uint32_t item_total;
+            block_exclusive_scan<BLOCK, uint32_t>(mine, wtmp, &item_total);
+            if (item_total) {
+                if (threadIdx.x == 0)
+                    s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)item_total);
+                __syncthreads();
+                uint64_t run = s_base;
+                // pass 2: place and write, round by round
+                for (uint64_t base = 0; base < pn; base += ROUND) {
+                    uint32_t kk[PITEMS], first[PITEMS];
+                    uint32_t c = 0;
+#pragma unroll
+                    for (int it = 0; it < PITEMS; it++) {
+                        const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
+                        kk[it] = (idx < pn) ? pkp[idx] : 0u;
+                    }
+#pragma unroll
+                    for (int it = 0; it < PITEMS; it++) {
+                        const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
+                        uint32_t f = 0;
+                        if (idx < pn) {
+                            const uint32_t k = kk[it];
+                            f = head[tb ? ((radix_hash(k) << bits) >> tshift) : 0u];
+                            for (uint32_t e = f; e; e = nxt[e - 1]) c += (hkey[e - 1] == k);
+                        }
+                        first[it] = f;
+                    }
+                    uint32_t rtotal;
+                    const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, wtmp, &rtotal);
+                    if (c) {
+                        uint64_t w = run + pre;
+#pragma unroll
+                        for (int it = 0; it < PITEMS; it++) {
+                            const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
+                            if (idx < pn && first[it]) {
+                                const uint32_t k = kk[it];
+                                uint32_t pval = 0;
+                                bool have = false;
+                                for (uint32_t e = first[it]; e; e = nxt[e - 1]) {
+                                    if (hkey[e - 1] == k) {
+                                        if (!have) {
+                                            pval = pvp[idx];
+                                            have = true;
+                                        }
+                                        if (w < out_cap) {
+                                            const uint32_t bval = bvp[e - 1];
+                                            o_key[w] = k;
+                                            o_rval[w] = build_r ? bval : pval;
+                                            o_sval[w] = build_r ? pval : bval;
+                                        }
+                                        w++;
+                                    }
                                 }
                             }
                         }
-                        c = nxt[j];
                     }
+                    run += rtotal;
                 }
-                __syncthreads();
-                const uint32_t staged = s_ocnt;
-                __syncthreads();
-                if (staged > (uint32_t)(OB - BLOCK)) flush(staged);
             }
         }
+        __syncthreads();  // the table is rebuilt by the next item; s_next is written
+        item = s_next;
+        __syncthreads();
     }
-    __syncthreads();
-    const uint32_t staged = s_ocnt;
-    __syncthreads();
-    flush(staged);
 }
 
 // ============================================================== launchers
@@ -710,12 +1023,21 @@ hipError_t launch_gen_s(uint32_t* key, uint32_t* val, uint64_t n_s, uint64_t n_r
 }
 
 constexpr int HIST_BLOCK = 1024;
-constexpr int SC_BLOCK = 512;
-constexpr int SC_ITEMS = 8;
+constexpr int SC_BLOCK = 1024;
+constexpr int SC_ITEMS = 4;
+constexpr int FL_BLOCK = 512;
+constexpr int FL_ITEMS = 8;
+constexpr int FL_BUF = 8192;
 
+// Allow dynamic LDS up to the device limit minus the kernel's static LDS.
 static hipError_t set_lds(const void* fn, size_t bytes)
 {
-    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    hipFuncAttributes attr;
+    hipError_t e = hipFuncGetAttributes(&attr, fn);
+    if (e != hipSuccess) return e;
+    const size_t stat = attr.sharedSizeBytes;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(bytes > stat ? bytes - stat : 0));
 }
 
 hipError_t kernels_init(int max_lds_bytes)
@@ -724,6 +1046,7 @@ hipError_t kernels_init(int max_lds_bytes)
     if ((e = set_lds((const void*)k_hist<HIST_BLOCK>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds((const void*)k_scatter<SC_BLOCK, SC_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds((const void*)k_bloom_build<1024>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds((const void*)k_filter_l1<FL_BLOCK, FL_ITEMS, FL_BUF>, max_lds_bytes)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -735,13 +1058,33 @@ hipError_t launch_hist(const uint32_t* key, uint64_t n_host, const uint64_t* n_d
     return hipGetLastError();
 }
 
+hipError_t launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = (int)std::min<uint64_t>(4096, (n + 255) / 256);
+    k_zero_u32<<<grid, 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce(const uint32_t* part, int G, int bits, int bits2, uint32_t* totals,
-                         uint32_t* l1part, hipStream_t st)
+                         uint32_t* l1h, hipStream_t st)
 {
     const int nbins = 1 << bits;
-    k_reduce_totals<<<ceil_div_i(nbins, 256), 256, 0, st>>>(part, G, nbins, totals);
-    const int n1 = (1 << (bits - bits2)) * G;
-    k_reduce_l1<<<ceil_div_i(n1, 256), 256, 0, st>>>(part, G, bits, bits2, l1part);
+    hipError_t e = launch_zero_u32(totals, (uint64_t)nbins, st);
+    if (e != hipSuccess) return e;
+    dim3 grid(ceil_div_i(nbins, 256), ceil_div_i(G, RCH));
+    k_reduce_totals<<<grid, 256, 0, st>>>(part, G, nbins, totals);
+    if (bits2 > 0) {
+        const int n1 = (1 << (bits - bits2)) * G;
+        k_reduce_l1<<<ceil_div_i(n1, 256), 256, 0, st>>>(part, G, bits, bits2, l1h);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_l1_offsets(const uint32_t* l1h, int G, int P1, int bits2, const uint64_t* sb_start,
+                             uint64_t* l1off, hipStream_t st)
+{
+    k_l1_offsets<<<ceil_div_i(P1, 64), 1024, 0, st>>>(l1h, G, P1, bits2, sb_start, l1off);
     return hipGetLastError();
 }
 
@@ -758,68 +1101,102 @@ hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* 
     return hipGetLastError();
 }
 
+// Write-combining geometry: tile T = SC_BLOCK * SC_ITEMS tuples, carry chunk
+// cs tuples (64 B up to 512 buckets, then 32 B / 16 B so the carry stays at
+// 64 KB of LDS).
+int scatter_carry(int nbk) { return nbk <= 512 ? 16 : (nbk <= 1024 ? 8 : 4); }
+
 size_t scatter_lds_bytes(int nbk)
 {
-    return (size_t)nbk * 8 + (size_t)SC_BLOCK * SC_ITEMS * 8 + (size_t)nbk * 8 +
+    const int cs = scatter_carry(nbk);
+    return (size_t)nbk * 8 + (size_t)nbk * cs * 8 + (size_t)SC_BLOCK * SC_ITEMS * 8 + (size_t)nbk * 12 +
            (SC_BLOCK / WAVE + 1) * 4;
+}
+
+hipError_t launch_l2_offsets(const uint32_t* part, int G, int bits, int W2, const uint64_t* sb_start,
+                             uint64_t* l2cur, hipStream_t st)
+{
+    const int nbins = 1 << bits;
+    k_l2_offsets<<<ceil_div_i(nbins, 256), 256, 0, st>>>(part, G, nbins, W2, sb_start, l2cur);
+    return hipGetLastError();
 }
 
 hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* okey, uint32_t* oval,
                           uint64_t out_cap, uint64_t n_host, const uint64_t* n_dev, int G, int mode,
-                          int bits, int bits2, const uint64_t* l1off, const uint64_t* sb_start,
-                          uint64_t* err, hipStream_t st)
+                          int bits, int p1, int W2, const uint64_t* l1off, const uint64_t* sb_start,
+                          const uint64_t* l2cur, uint64_t* err, hipStream_t st)
 {
-    const int nbk = (mode == 0) ? (1 << (bits - bits2)) : (1 << bits2);
-    const int grid = (mode == 0) ? G : (1 << (bits - bits2));
+    const int nbk = (mode == 0) ? (1 << p1) : (1 << (bits - p1));
+    const int grid = (mode == 0) ? G : (1 << p1) * W2;
     k_scatter<SC_BLOCK, SC_ITEMS><<<grid, SC_BLOCK, scatter_lds_bytes(nbk), st>>>(
-        ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, bits2, l1off, sb_start, err);
+        ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, scatter_carry(nbk), l1off,
+        sb_start, l2cur, err);
     return hipGetLastError();
 }
 
 hipError_t launch_bloom_build(const uint32_t* rkey, const uint64_t* r_start, int bits,
-                              const BloomGeom& g, uint32_t* filter, hipStream_t st)
+                              const BloomGeom& g, uint64_t* filter, hipStream_t st)
 {
     const int nseg = 1 << g.seg_bits;
-    k_bloom_build<1024><<<nseg, 1024, (size_t)g.wps * 4, st>>>(rkey, r_start, bits, g, filter);
+    k_bloom_build<1024><<<nseg, 1024, (size_t)g.wps * 8, st>>>(rkey, r_start, bits, g, filter);
     return hipGetLastError();
 }
 
-hipError_t launch_bloom_filter(const uint32_t* skey, const uint32_t* sval, uint64_t n,
-                               const uint32_t* filter, const BloomGeom& g, uint32_t* okey,
-                               uint32_t* oval, uint64_t out_cap, uint64_t* count, uint64_t* err,
-                               int num_cu, hipStream_t st)
+hipError_t launch_bloom_prefilter(const uint32_t* skey, const uint32_t* sval, uint64_t n,
+                                  const uint64_t* filter, const BloomGeom& g, uint32_t* okey,
+                                  uint32_t* oval, uint64_t out_cap, uint64_t* count, uint64_t* err,
+                                  int num_cu, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     constexpr int B = 512, I = 8;
     const int grid = (int)std::min<uint64_t>((uint64_t)num_cu * 8, (n + B * I - 1) / (B * I));
-    k_bloom_filter<B, I><<<grid, B, 0, st>>>(skey, sval, n, filter, g, okey, oval, out_cap, count, err);
+    k_bloom_prefilter<B, I><<<grid, B, 0, st>>>(skey, sval, n, filter, g, okey, oval, out_cap, count, err);
     return hipGetLastError();
 }
 
-hipError_t launch_plan_probe(const uint64_t* r_start, const uint64_t* s_start, int bits,
-                             uint32_t tile, uint32_t slice, HeavyItem* items, uint64_t item_cap,
-                             uint64_t* n_items, uint64_t* err, hipStream_t st)
+hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const uint64_t* sb_start, int bits,
+                            int bits2, const uint64_t* filter, const BloomGeom& g, uint32_t* okey,
+                            uint32_t* oval, uint64_t out_cap, uint32_t* s_fill, uint64_t* count,
+                            uint64_t* err, int num_cu, hipStream_t st)
+{
+    const int P2 = 1 << bits2;
+    const size_t lds = (size_t)P2 * 8 + (size_t)FL_BUF * 8 + (size_t)P2 * 8;
+    const int W = std::max(1, num_cu / 8) * 2;  // workgroups per XCD
+    k_filter_l1<FL_BLOCK, FL_ITEMS, FL_BUF><<<8 * W, FL_BLOCK, lds, st>>>(
+        ikey, ival, sb_start, bits, bits2, filter, g, okey, oval, out_cap, s_fill, count, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan(const uint64_t* r_start, const uint64_t* s_start, const uint32_t* s_cnt, int bits,
+                       uint32_t tile, uint32_t slice, uint32_t max_slices, uint32_t* counts,
+                       uint64_t* item_start, uint64_t* partials, HeavyItem* items, uint64_t item_cap,
+                       uint64_t* err, hipStream_t st)
 {
     const int nbins = 1 << bits;
-    k_plan_probe<<<ceil_div_i(nbins, 256), 256, 0, st>>>(r_start, s_start, nbins, tile, slice, items,
-                                                         item_cap, n_items, err);
+    k_plan_count<<<ceil_div_i(nbins, 256), 256, 0, st>>>(r_start, s_start, s_cnt, nbins, tile, slice,
+                                                         max_slices, counts);
+    hipError_t e = launch_scan(counts, (uint64_t)nbins, item_start, partials, st);
+    if (e != hipSuccess) return e;
+    k_plan_fill<<<ceil_div_i(nbins, 256), 256, 0, st>>>(r_start, s_start, s_cnt, nbins, tile, slice,
+                                                        max_slices, item_start, item_cap, items, err);
     return hipGetLastError();
 }
 
 hipError_t launch_probe(int tile, const uint32_t* rk, const uint32_t* rv, const uint64_t* r_start,
-                        const uint32_t* sk, const uint32_t* sv, const uint64_t* s_start, int bits,
-                        uint32_t slice, const HeavyItem* items, const uint64_t* n_items,
-                        uint64_t* work_ctr, uint64_t* out_count, uint64_t out_cap, uint32_t* o_key,
-                        uint32_t* o_rval, uint32_t* o_sval, int num_cu, hipStream_t st)
+                        const uint32_t* sk, const uint32_t* sv, const uint64_t* s_start,
+                        const uint32_t* s_cnt, int bits, uint32_t slice, const HeavyItem* items,
+                        const uint64_t* n_items, uint64_t item_cap, uint64_t* work_ctr, uint64_t* out_count,
+                        uint64_t out_cap, uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, int num_cu,
+                        hipStream_t st)
 {
     if (tile == 8192) {
-        k_probe<1024, 8192, 4><<<num_cu, 1024, 0, st>>>(rk, rv, r_start, sk, sv, s_start, bits, slice,
-                                                         items, n_items, work_ctr, out_count, out_cap,
-                                                         o_key, o_rval, o_sval);
+        k_probe<512, 8192, 8><<<num_cu * 2, 512, 0, st>>>(rk, rv, r_start, sk, sv, s_start, s_cnt, bits,
+                                                          slice, items, n_items, item_cap, work_ctr,
+                                                          out_count, out_cap, o_key, o_rval, o_sval);
     } else if (tile == 4096) {
-        k_probe<512, 4096, 4><<<num_cu * 2, 512, 0, st>>>(rk, rv, r_start, sk, sv, s_start, bits, slice,
-                                                          items, n_items, work_ctr, out_count, out_cap,
-                                                          o_key, o_rval, o_sval);
+        k_probe<256, 4096, 8><<<num_cu * 4, 256, 0, st>>>(rk, rv, r_start, sk, sv, s_start, s_cnt, bits,
+                                                          slice, items, n_items, item_cap, work_ctr,
+                                                          out_count, out_cap, o_key, o_rval, o_sval);
     } else {
         return hipErrorInvalidValue;
     }

@@ -100,6 +100,16 @@ typedef struct bj_params {
     int32_t inputs_on_device;       /* 1: r_key.. are device pointers; 0: host         */
     int32_t collect_phase_times;    /* 1: fill bj_result.phase_ms                      */
     int32_t s_chunks;               /* argv[12]; accepted, S is processed in one pass  */
+    int32_t bf_stage;               /* where the filter drops S tuples: 1 = before S is
+                                       partitioned (the reference's order); 2 = after
+                                       the first-level pass, one filter slice per
+                                       bucket resident in an XCD's L2; 0 = auto (1 when
+                                       the filter fits the 256 MB MALL, else 2)        */
+    int32_t pass1_bits;             /* radix bits of the first physical scatter pass
+                                       (the rest go to the second); -1 = auto (half).
+                                       The resulting sub-bucket layout does not depend
+                                       on it; partition_size is the logical first
+                                       level, whose buckets are unions of sub-buckets  */
 } bj_params;
 
 /* ---- join result ------------------------------------------------------- */
@@ -121,6 +131,8 @@ typedef struct bj_result {
     uint64_t bf_words;        /* filter size in 32-bit words (0 = Bloom off)         */
     int32_t  tile;            /* LDS build tile (tuples)                             */
     uint64_t heavy_items;     /* vlsplit work items for oversized sub-buckets        */
+    int32_t  bf_stage;        /* 1 or 2 as in bj_params (0 = Bloom off)              */
+    int32_t  pass1_bits;      /* radix bits of the first physical scatter pass       */
 } bj_result;
 
 /* ---- the plan a join would use (no device needed) ---------------------- */
@@ -133,6 +145,7 @@ typedef struct bj_plan_info {
     uint64_t capacity;           /* output row capacity                        */
     uint64_t workspace_bytes;    /* device memory a context needs for the join,
                                     excluding device-resident inputs           */
+    int32_t  bf_stage;           /* 1 or 2 (0 = Bloom off)                     */
 } bj_plan_info;
 
 typedef struct bj_ctx bj_ctx;
@@ -161,6 +174,16 @@ int  bj_join(bj_ctx *ctx, const bj_params *p,
              const uint32_t *r_key, const uint32_t *r_val,
              const uint32_t *s_key, const uint32_t *s_val,
              bj_result *out);
+
+/* Two-level radix partition of one relation (device pointers), with the radix
+ * bits bj_join would use for p (r_size = n).  Writes the partitioned tuples to
+ * out_key/out_val (n entries each) and the sub-bucket starts to out_start
+ * (2^(bits1+bits2) + 1 entries; bucket b of a key = the top bits of
+ * lowbias32(key)).  Counterpart of partial_jointable_gpu_partition_r_only
+ * (src/vlsplit_hash_kernel.h:40-46). */
+int  bj_partition(bj_ctx *ctx, const bj_params *p, const uint32_t *key,
+                  const uint32_t *val, int64_t n, uint32_t *out_key,
+                  uint32_t *out_val, uint64_t *out_start);
 
 /* Copy the first n rows of a result to host arrays (n <= rows_written). */
 int  bj_copy_rows_to_host(bj_ctx *ctx, const bj_result *res,

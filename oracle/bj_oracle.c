@@ -320,41 +320,36 @@ long long orc_nested_loop(const uint32_t *r_key, const uint32_t *r_val, long lon
 /* ------------------------------------------------------------------------- */
 uint32_t orc_radix_hash(uint32_t key) { return orc_lowbias32(key); }
 
-static void bloom_addr(uint32_t key, int seg_bits, uint64_t words_per_seg, int k,
-                       uint64_t *word, uint32_t *mask)
+static void bloom_addr(uint32_t key, int seg_bits, uint64_t wps, int k, uint64_t *block,
+                       uint64_t *mask)
 {
     const uint32_t h = orc_lowbias32(key);
     const uint64_t seg = seg_bits ? (uint64_t)(h >> (32 - seg_bits)) : 0;
-    const uint32_t g = seg_bits ? (h << seg_bits) : h;
-    const uint64_t nblk = words_per_seg / 8;
-    const uint64_t blk = ((uint64_t)g * nblk) >> 32;
+    const uint32_t rest = seg_bits ? (h << seg_bits) : h;
+    *block = seg * wps + (((uint64_t)rest * wps) >> 32);
     const uint32_t h2 = orc_lowbias32(key ^ 0x9E3779B9u);
     const uint32_t h3 = orc_lowbias32(h2 ^ 0x85EBCA6Bu);
+    uint64_t m = 0;
     for (int i = 0; i < k; i++) {
-        uint32_t bit = (i < 4) ? ((h2 >> (8 * i)) & 255u) : ((h3 >> (8 * (i - 4))) & 255u);
-        word[i] = seg * words_per_seg + blk * 8 + (bit >> 5);
-        mask[i] = 1u << (bit & 31);
+        uint32_t bit = (i < 5) ? ((h2 >> (6 * i)) & 63u) : ((h3 >> (6 * (i - 5))) & 63u);
+        m |= 1ULL << bit;
     }
+    *mask = m;
 }
 
-void orc_bloom_build(const uint32_t *keys, long long n, int seg_bits,
-                     uint64_t words_per_seg, int k, uint32_t *filter)
+void orc_bloom_build(const uint32_t *keys, long long n, int seg_bits, uint64_t wps, int k,
+                     uint64_t *filter)
 {
-    uint64_t word[8];
-    uint32_t mask[8];
     for (long long i = 0; i < n; i++) {
-        bloom_addr(keys[i], seg_bits, words_per_seg, k, word, mask);
-        for (int b = 0; b < k; b++) filter[word[b]] |= mask[b];
+        uint64_t b, m;
+        bloom_addr(keys[i], seg_bits, wps, k, &b, &m);
+        filter[b] |= m;
     }
 }
 
-int orc_bloom_query(const uint32_t *filter, uint32_t key, int seg_bits,
-                    uint64_t words_per_seg, int k)
+int orc_bloom_query(const uint64_t *filter, uint32_t key, int seg_bits, uint64_t wps, int k)
 {
-    uint64_t word[8];
-    uint32_t mask[8];
-    bloom_addr(key, seg_bits, words_per_seg, k, word, mask);
-    for (int b = 0; b < k; b++)
-        if (!(filter[word[b]] & mask[b])) return 0;
-    return 1;
+    uint64_t b, m;
+    bloom_addr(key, seg_bits, wps, k, &b, &m);
+    return (filter[b] & m) == m;
 }

@@ -60,12 +60,11 @@ long long orc_nested_loop(const uint32_t *r_key, const uint32_t *r_val, long lon
 
 /* ---- radix / bloom addressing (restated from the product's spec) ---- */
 uint32_t orc_radix_hash(uint32_t key);
-/* Build a blocked Bloom filter segment layout: 2^seg_bits segments of
- * words_per_seg 32-bit words (multiple of 8); k bits per key in one 256-bit block. */
-void orc_bloom_build(const uint32_t *keys, long long n, int seg_bits,
-                     uint64_t words_per_seg, int k, uint32_t *filter);
-int  orc_bloom_query(const uint32_t *filter, uint32_t key, int seg_bits,
-                     uint64_t words_per_seg, int k);
+/* Register-blocked Bloom filter: 2^seg_bits segments of wps 64-bit blocks;
+ * all k bits of a key in one 64-bit block. */
+void orc_bloom_build(const uint32_t *keys, long long n, int seg_bits, uint64_t wps, int k,
+                     uint64_t *filter);
+int  orc_bloom_query(const uint64_t *filter, uint32_t key, int seg_bits, uint64_t wps, int k);
 
 #ifdef __cplusplus
 }

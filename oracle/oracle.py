@@ -67,10 +67,11 @@ def lib():
                                       ctypes.c_longlong]
         L.orc_radix_hash.restype = ctypes.c_uint32
         L.orc_radix_hash.argtypes = [ctypes.c_uint32]
+        _u64p = ctypes.POINTER(ctypes.c_uint64)
         L.orc_bloom_build.argtypes = [_u32p, ctypes.c_longlong, ctypes.c_int, ctypes.c_uint64,
-                                      ctypes.c_int, _u32p]
+                                      ctypes.c_int, _u64p]
         L.orc_bloom_query.restype = ctypes.c_int
-        L.orc_bloom_query.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64,
+        L.orc_bloom_query.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64,
                                       ctypes.c_int]
         _lib = L
     return _lib
@@ -197,16 +198,16 @@ def radix_hash(key: int) -> int:
     return int(lib().orc_radix_hash(key))
 
 
-def bloom_build(keys: np.ndarray, seg_bits: int, words_per_seg: int, k: int) -> np.ndarray:
+def bloom_build(keys: np.ndarray, seg_bits: int, wps: int, k: int) -> np.ndarray:
+    """Register-blocked filter: 2^seg_bits segments of wps 64-bit blocks."""
     keys = np.ascontiguousarray(keys, dtype=np.uint32)
-    f = np.zeros((1 << seg_bits) * words_per_seg, np.uint32)
-    lib().orc_bloom_build(_p32(keys), len(keys), seg_bits, words_per_seg, k, _p32(f))
+    f = np.zeros((1 << seg_bits) * wps, np.uint64)
+    lib().orc_bloom_build(_p32(keys), len(keys), seg_bits, wps, k,
+                          f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     return f
 
 
-def bloom_query(filt: np.ndarray, keys: np.ndarray, seg_bits: int, words_per_seg: int,
-                k: int) -> np.ndarray:
+def bloom_query(filt: np.ndarray, keys: np.ndarray, seg_bits: int, wps: int, k: int) -> np.ndarray:
     L = lib()
-    fp = _p32(filt)
-    return np.array([L.orc_bloom_query(fp, int(x), seg_bits, words_per_seg, k) for x in keys],
-                    dtype=bool)
+    fp = filt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    return np.array([L.orc_bloom_query(fp, int(x), seg_bits, wps, k) for x in keys], dtype=bool)

@@ -47,6 +47,17 @@ def gpu_join(ctx, rk, rv, sk, sv, **kw):
     return res, O.sort_rows(k, a, b)
 
 
+def radix_hash_np(x):
+    """Vectorised lowbias32 (checked against the oracle's scalar version below)."""
+    x = np.asarray(x, np.uint32).copy()
+    x ^= x >> np.uint32(16)
+    x *= np.uint32(0x7FEB352D)
+    x ^= x >> np.uint32(15)
+    x *= np.uint32(0x846CA68B)
+    x ^= x >> np.uint32(16)
+    return x
+
+
 def oracle_rows(rk, rv, sk, sv):
     n, k, a, b = O.cpu_join(rk, rv, sk, sv)
     return n, O.sort_rows(k, a, b)
@@ -55,6 +66,9 @@ def oracle_rows(rk, rv, sk, sv):
 VARIANTS = [
     dict(),
     dict(use_bf=1, bf_bits_per_key=10.0, num_hashes=3),
+    dict(use_bf=1, bf_stage=1, radix_bits=12),
+    dict(use_bf=1, bf_stage=2, radix_bits=12),
+    dict(use_bf=1, bf_stage=2, radix_bits=6),
     dict(radix_bits=1),
     dict(radix_bits=4, tile=4096),
     dict(partition_size=64, second_partition_size=16, use_bf=10),
@@ -90,6 +104,9 @@ def test_generator_matches_oracle(gpu_ctx):
 @pytest.mark.parametrize("n_r,n_s,ratio,kw", [
     (1 << 20, 1 << 20, 0.1, dict()),
     (1 << 20, 1 << 20, 0.1, dict(use_bf=1, bf_bits_per_key=10.0, num_hashes=3)),
+    (1 << 21, 1 << 21, 0.1, dict(use_bf=1, bf_stage=2, partition_size=512, radix_bits=14,
+                                 bf_bits_per_key=10.0, num_hashes=3)),
+    (1 << 21, 3 << 20, 0.2, dict(use_bf=1, bf_stage=1, partition_size=512, radix_bits=14)),
     (300_000, 2_000_000, 0.5, dict(use_bf=1, bf_bits_per_key=12.0, num_hashes=4)),
     (2_000_000, 300_000, 1.0, dict(tile=4096)),
     (1_000_000, 1_000_000, 0.1, dict(radix_bits=2)),           # vlsplit: chunks x slices
@@ -214,3 +231,43 @@ def test_gpu_main_reference_files(tmp_path):
     k, a, b = bj.read_output(str(opath))
     n, expect = oracle_rows(z["r_key"], rv, z["s_key"], sv)
     assert np.array_equal(np.stack([k, a, b], 1), expect)
+
+
+@pytest.mark.parametrize("n,kw", [
+    (40, dict(radix_bits=12)),                          # most buckets empty
+    (100_000, dict(radix_bits=11)),                     # single level, 2048-way
+    (1_000_000, dict(partition_size=512, radix_bits=14)),
+    (777_777, dict(partition_size=1024, radix_bits=15)),
+    (3_000_000, dict(partition_size=64, second_partition_size=64)),
+    (0, dict(radix_bits=8)),
+])
+def test_partition_kernels(gpu_ctx, n, kw):
+    """Partition parity: a permutation of the input, every tuple in its bucket
+    (top bits of lowbias32(key)), bucket sizes equal to a CPU histogram."""
+    import torch
+    rng = np.random.default_rng(n + 1)
+    key = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    if n > 10:
+        key[: n // 10] = key[0]                           # a heavy key
+    val = np.arange(n, dtype=np.uint32)
+    p = bj.default_params(**kw)
+    info = bj.plan(bj.default_params(r_size=n, s_size=n, **kw))
+    bits = info.bits1 + info.bits2
+    dk, dv = to_dev(key), to_dev(val)
+    ok, ov = dev_empty(n), dev_empty(n)
+    st = torch.zeros((1 << bits) + 1, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    gpu_ctx.partition(p, dk, dv, n, ok, ov, st)
+    start = st.cpu().numpy().astype(np.uint64)
+    hk, hv = to_host(ok, n), to_host(ov, n)
+    assert np.array_equal(np.sort(hv), val)                  # a permutation (values are row ids)
+    assert np.array_equal(hk, key[hv])                       # keys travel with their values
+    hk32 = radix_hash_np(key)
+    assert all(int(hk32[i]) == O.radix_hash(int(key[i])) for i in range(min(n, 200)))
+    h = hk32.astype(np.uint64) >> np.uint64(32 - bits)
+    hist = np.bincount(h.astype(np.int64), minlength=1 << bits)
+    assert np.array_equal(np.diff(start), hist)              # bucket sizes
+    assert start[0] == 0 and start[-1] == n
+    hb = radix_hash_np(hk).astype(np.uint64) >> np.uint64(32 - bits)
+    pos = np.arange(n, dtype=np.uint64)
+    assert np.all((start[hb.astype(np.int64)] <= pos) & (pos < start[hb.astype(np.int64) + 1]))

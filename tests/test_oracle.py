@@ -135,15 +135,24 @@ def test_generator_reference_datagen_files_roundtrip(tmp_path):
     assert O.sanity_count(rk, sk) == 3000 == O.ref_sanity_count(rk, sk)[0]
 
 
-def test_bloom_no_false_negatives_and_fpr():
+@pytest.mark.parametrize("bpk,k,lo,hi", [(10, 3, 0.012, 0.035), (12, 4, 0.005, 0.025),
+                                         (4, 2, 0.08, 0.20)])
+def test_bloom_no_false_negatives_and_fpr(bpk, k, lo, hi):
+    """Register-blocked (64-bit) filter: no false negatives; the false-positive rate
+    matches the Poisson mixture over keys per block, sum_j P(j) (1-(63/64)^(k j))^k,
+    computed here independently of the filter code."""
+    import math
     n = 200000
     rk, _ = O.gen_r(7, n)
-    seg_bits, k = 3, 3
-    words = n * 10 // 32
-    wps = ((words >> seg_bits) + 7) // 8 * 8
+    seg_bits = 3
+    blocks = n * bpk // 64
+    wps = max(1, blocks >> seg_bits)
     f = O.bloom_build(rk, seg_bits, wps, k)
     assert O.bloom_query(f, rk[:5000], seg_bits, wps, k).all()
     probe = (np.arange(20000, dtype=np.uint64) + 2**31).astype(np.uint32)
     fpr = O.bloom_query(f, probe, seg_bits, wps, k).mean()
-    # blocked filter, 10 bits/key, k=3: classic 1.7%, blocking adds a little
-    assert 0.005 < fpr < 0.05
+    lam = n / (wps << seg_bits)
+    theory = sum(math.exp(-lam) * lam ** j / math.factorial(j) *
+                 (1 - (63 / 64) ** (k * j)) ** k for j in range(60))
+    assert lo < theory < hi
+    assert abs(fpr - theory) < 0.25 * theory + 0.003
