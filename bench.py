@@ -42,7 +42,7 @@ WORKLOADS = {
 
 def algorithmic_bytes(kernel: str, n_r: int, n_s: int, m: int, m_s: int, bf_bytes: int,
                       bloom: bool, bf_stage: int = 1) -> float:
-    """Compulsory streamed bytes of one launch of a kernel (see DESIGN.md):
+    """Compulsory streamed bytes of one launch of a kernel:
     a partition pass reads and writes 8 B per tuple, build+probe read 8 B per R
     and per S' tuple and write 12 B per row, the Bloom build reads 4 B per R key
     and writes the filter once, the Bloom filter reads 8 B per S tuple and writes
@@ -62,7 +62,9 @@ def algorithmic_bytes(kernel: str, n_r: int, n_s: int, m: int, m_s: int, bf_byte
 
 
 def total_algorithmic_bytes(n_r, n_s, m, m_s, bf_bytes, bloom) -> float:
-    """Whole-join algorithmic bytes, two partition passes (BASELINE.md formulas)."""
+    """Whole-join algorithmic bytes with two partition passes per relation: 40 B per
+    tuple without the filter; with it, 44 B per R tuple plus the filter, 8 B per S
+    key tested, 48 B per surviving S tuple, and 12 B per output row."""
     if bloom:
         return 44.0 * n_r + bf_bytes + 8.0 * n_s + 48.0 * m_s + 12.0 * m
     return 40.0 * (n_r + n_s) + 12.0 * m

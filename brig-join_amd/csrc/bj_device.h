@@ -23,9 +23,12 @@ __host__ __device__ __forceinline__ uint32_t lowbias32(uint32_t x)
     return x;
 }
 
-// Radix hash: partition bucket = top `bits` bits; the LDS hash-table bucket of
-// the build/probe kernel takes the bits just below them.
-__device__ __forceinline__ uint32_t radix_hash(uint32_t key) { return lowbias32(key); }
+// Radix hash: Fibonacci hashing, key * 2^32/phi (odd), a bijection on uint32 in
+// one multiply.  Partition bucket = its top `bits` bits (which depend on every
+// key bit); the LDS hash-table bucket of the build/probe kernel takes the bits
+// just below them.
+constexpr uint32_t RADIX_MUL = 0x9E3779B1u;
+__host__ __device__ __forceinline__ uint32_t radix_hash(uint32_t key) { return key * RADIX_MUL; }
 
 // ---------------------------------------------------------------- generator
 // 4 rounds of (add, odd multiply, xor-shift right) mod 2^w: each step is a
@@ -69,21 +72,28 @@ struct BloomGeom {
 
 __host__ __device__ __forceinline__ uint64_t bloom_block(uint32_t key, const BloomGeom& g)
 {
-    const uint32_t h = lowbias32(key);
+    const uint32_t h = radix_hash(key);
     const uint64_t seg = g.seg_bits ? (uint64_t)(h >> (32 - g.seg_bits)) : 0ull;
     const uint32_t rest = g.seg_bits ? (h << g.seg_bits) : h;
     return seg * g.wps + (((uint64_t)rest * g.wps) >> 32);
 }
 
+// The k bit positions are 6-bit fields of a second multiplicative mix (fields
+// taken from the top, xor-folded so the low fields see the high key bits).
 __host__ __device__ __forceinline__ uint64_t bloom_mask(uint32_t key, int k)
 {
-    const uint32_t h2 = lowbias32(key ^ 0x9E3779B9u);
-    const uint32_t h3 = lowbias32(h2 ^ 0x85EBCA6Bu);
+    uint32_t h2 = (key ^ 0x5BD1E995u) * 0xC2B2AE3Du;
+    h2 ^= h2 >> 15;
+    uint32_t h3 = 0;
+    if (k > 5) {
+        h3 = h2 * 0x27D4EB2Fu;
+        h3 ^= h3 >> 15;
+    }
     uint64_t m = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         if (i < k) {
-            const uint32_t bit = (i < 5) ? ((h2 >> (6 * i)) & 63u) : ((h3 >> (6 * (i - 5))) & 63u);
+            const uint32_t bit = (i < 5) ? ((h2 >> (26 - 6 * i)) & 63u) : ((h3 >> (26 - 6 * (i - 5))) & 63u);
             m |= 1ull << bit;
         }
     }

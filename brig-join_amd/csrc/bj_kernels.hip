@@ -311,7 +311,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
     const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ ival,
     uint32_t* __restrict__ okey, uint32_t* __restrict__ oval, uint64_t out_cap,
     uint64_t n_host, const uint64_t* __restrict__ n_dev, int G, int mode, int bits, int p1, int W2,
-    int cs, const uint64_t* __restrict__ l1off, const uint64_t* __restrict__ sb_start,
+    int cs_log2, const uint64_t* __restrict__ l1off, const uint64_t* __restrict__ sb_start,
     const uint64_t* __restrict__ l2cur, uint64_t* __restrict__ err)
 {
     constexpr int T = BLOCK * ITEMS;
@@ -319,18 +319,19 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
     const int nbk = (mode == 0) ? (1 << p1) : (1 << b2);
     const int shift = (mode == 0) ? (32 - p1) : (32 - bits);
     const uint32_t bmask = (uint32_t)nbk - 1u;
-    const uint64_t amask = ~(uint64_t)(cs - 1);
+    const uint32_t cs = 1u << cs_log2;
 
     extern __shared__ uint64_t lds64[];
-    uint64_t* cur = lds64;                                 // [nbk]
+    uint64_t* cur = lds64;                                 // [nbk] next global position
     uint32_t* ck = (uint32_t*)(cur + nbk);                 // [nbk * cs] carried keys
     uint32_t* cv = ck + nbk * cs;                          // [nbk * cs] carried values
     uint32_t* skey = cv + nbk * cs;                        // [T]
     uint32_t* sval = skey + T;                             // [T]
-    uint32_t* cnt = sval + T;                              // [nbk]
-    uint32_t* off = cnt + nbk;                             // [nbk]
-    uint32_t* ccnt = off + nbk;                            // [nbk]
-    uint32_t* wtmp = ccnt + nbk;                           // [BLOCK/64 + 1]
+    uint32_t* cnt = sval + T;                              // [nbk] tile tuples
+    uint32_t* off = cnt + nbk;                             // [nbk] staging offset
+    uint32_t* ccnt = off + nbk;                            // [nbk] carried tuples
+    uint32_t* nw = ccnt + nbk;                             // [nbk] tuples written this tile
+    uint32_t* wtmp = nw + nbk;                             // [BLOCK/64 + 1]
 
     uint64_t lo, hi;
     if (mode == 0) {
@@ -353,19 +354,22 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
     // cur/ccnt are read by other threads; an empty range skips the tile loop's barriers
     __syncthreads();
 
+    const uint32_t* kin = ikey + lo;
+    const uint32_t* vin = ival + lo;
+    const uint32_t len = (uint32_t)(hi - lo);  // ranges are below 2^31 tuples
     uint32_t k[ITEMS], v[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const uint64_t i = lo + (uint64_t)it * BLOCK + threadIdx.x;
-        if (i < hi) {
-            k[it] = __builtin_nontemporal_load(&ikey[i]);
-            v[it] = __builtin_nontemporal_load(&ival[i]);
+        const uint32_t i = (uint32_t)it * BLOCK + threadIdx.x;
+        if (i < len) {
+            k[it] = __builtin_nontemporal_load(&kin[i]);
+            v[it] = __builtin_nontemporal_load(&vin[i]);
         }
     }
     bool bad = false;
 
-    for (uint64_t tile = lo; tile < hi; tile += T) {
-        const int tn = (int)min((uint64_t)T, hi - tile);
+    for (uint32_t tile = 0; tile < len; tile += T) {
+        const int tn = (int)min((uint32_t)T, len - tile);
         for (int b = threadIdx.x; b < nbk; b += BLOCK) cnt[b] = 0u;
         __syncthreads();
         uint32_t bk[ITEMS], rk[ITEMS];
@@ -379,6 +383,12 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
         }
         __syncthreads();
         lds_exclusive_scan<BLOCK>(cnt, off, nbk, wtmp);
+        // tuples of bucket b written this tile: up to its last whole chunk
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
+            const uint32_t a0 = (uint32_t)cur[b] & (cs - 1u);
+            const uint32_t e = a0 + ccnt[b] + cnt[b];
+            nw[b] = e >= cs ? (e & ~(cs - 1u)) - a0 : 0u;
+        }
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
@@ -390,46 +400,39 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
             }
         }
         // prefetch the next tile while this one is written out
-        const uint64_t next = tile + T;
+        const uint32_t next = tile + T;
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
-            const uint64_t i = next + (uint64_t)it * BLOCK + threadIdx.x;
-            if (i < hi) {
-                k[it] = __builtin_nontemporal_load(&ikey[i]);
-                v[it] = __builtin_nontemporal_load(&ival[i]);
+            const uint32_t i = next + (uint32_t)it * BLOCK + threadIdx.x;
+            if (i < len) {
+                k[it] = __builtin_nontemporal_load(&kin[i]);
+                v[it] = __builtin_nontemporal_load(&vin[i]);
             }
         }
         __syncthreads();
-        // phase A: carried tuples below the write end go out
-        for (int idx = threadIdx.x; idx < nbk * cs; idx += BLOCK) {
-            const int b = idx / cs, i = idx - b * cs;
-            const uint32_t cc = ccnt[b];
-            if ((uint32_t)i < cc) {
-                const uint64_t c0 = cur[b];
-                const uint64_t e = c0 + cc + cnt[b];
-                const uint64_t wend = max(e & amask, c0);
-                const uint64_t g = c0 + (uint64_t)i;
-                if (g < wend) {
-                    if (g < out_cap) {
-                        okey[g] = ck[idx];
-                        oval[g] = cv[idx];
-                    } else {
-                        bad = true;
-                    }
+        // phase A: carried tuples inside the written range go out
+        for (int idx = threadIdx.x; idx < (nbk << cs_log2); idx += BLOCK) {
+            const int b = idx >> cs_log2;
+            const uint32_t i = (uint32_t)idx & (cs - 1u);
+            if (i < ccnt[b] && i < nw[b]) {
+                const uint64_t g = cur[b] + i;
+                if (g < out_cap) {
+                    okey[g] = ck[idx];
+                    oval[g] = cv[idx];
+                } else {
+                    bad = true;
                 }
             }
         }
         __syncthreads();
-        // phase B: the tile's tuples go out, or into the carry below CS
+        // phase B: the tile's tuples go out, or into the carry
         for (int s = threadIdx.x; s < tn; s += BLOCK) {
             const uint32_t kk = skey[s];
             const uint32_t b = (radix_hash(kk) >> shift) & bmask;
-            const uint64_t c0 = cur[b];
-            const uint32_t cc = ccnt[b];
-            const uint64_t e = c0 + cc + cnt[b];
-            const uint64_t wend = max(e & amask, c0);
-            const uint64_t g = c0 + cc + (uint64_t)(s - (int)off[b]);
-            if (g < wend) {
+            const uint32_t rel = ccnt[b] + (uint32_t)s - off[b];
+            const uint32_t w_b = nw[b];
+            if (rel < w_b) {
+                const uint64_t g = cur[b] + rel;
                 if (g < out_cap) {
                     okey[g] = kk;
                     oval[g] = sval[s];
@@ -437,26 +440,24 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
                     bad = true;
                 }
             } else {
-                const uint32_t slot = (uint32_t)(g - wend);
-                ck[b * cs + slot] = kk;
-                cv[b * cs + slot] = sval[s];
+                const uint32_t slot = (b << cs_log2) + rel - w_b;
+                ck[slot] = kk;
+                cv[slot] = sval[s];
             }
         }
         __syncthreads();
         for (int b = threadIdx.x; b < nbk; b += BLOCK) {
-            const uint64_t c0 = cur[b];
-            const uint64_t e = c0 + ccnt[b] + cnt[b];
-            const uint64_t wend = max(e & amask, c0);
-            ccnt[b] = (uint32_t)(e - wend);
-            cur[b] = wend;
+            cur[b] += nw[b];
+            ccnt[b] = ccnt[b] + cnt[b] - nw[b];
         }
         __syncthreads();
     }
     // the remaining carries (partial chunks at the range ends)
-    for (int idx = threadIdx.x; idx < nbk * cs; idx += BLOCK) {
-        const int b = idx / cs, i = idx - b * cs;
-        if ((uint32_t)i < ccnt[b]) {
-            const uint64_t g = cur[b] + (uint64_t)i;
+    for (int idx = threadIdx.x; idx < (nbk << cs_log2); idx += BLOCK) {
+        const int b = idx >> cs_log2;
+        const uint32_t i = (uint32_t)idx & (cs - 1u);
+        if (i < ccnt[b]) {
+            const uint64_t g = cur[b] + i;
             if (g < out_cap) {
                 okey[g] = ck[idx];
                 oval[g] = cv[idx];
@@ -614,13 +615,12 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_prefilter(
 // the filter.  Workgroups are dealt to XCDs round-robin (workgroup i runs on
 // XCD i % 8), so XCD x walks buckets x, x+8, ... with all its workgroups on the
 // same bucket at the same time: that bucket's filter slice stays in the XCD's
-// L2 while its S tuples stream past (loaded non-temporally).
-// Survivors are compacted into an LDS buffer (wave ballot + one LDS atomic per
-// wave) and counted per sub-bucket.  A flush reserves each sub-bucket's rows
-// with one global atomic and writes them into the sub-bucket's region of the S
-// layout (sub-bucket sb gets positions sb_start[sb] + [0, s_fill[sb])), so no
-// second pass over S is needed.  Values are read only for survivors.
-template <int BLOCK, int ITEMS, int BUF>
+// L2 while its S tuples stream past (loaded non-temporally).  Survivors are
+// counting-sorted by sub-bucket in LDS per tile, reserved with one atomic per
+// (tile, sub-bucket) and written, compacted, into their sub-bucket's region of
+// the S layout (sub-bucket sb gets positions sb_start[sb] + [0, s_fill[sb])),
+// so no second pass over S is needed.  Values are read only for survivors.
+template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_filter_l1(
     const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ ival,
     const uint64_t* __restrict__ sb_start, int bits, int bits2, const uint64_t* __restrict__ filter,
@@ -629,57 +629,24 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
 {
     constexpr int T = BLOCK * ITEMS;
     constexpr int NXCD = 8;
-    static_assert(BUF >= T, "the buffer holds at least one tile");
     const int P1 = 1 << (bits - bits2), P2 = 1 << bits2;
     const int shift = 32 - bits;
     const uint32_t bmask = (uint32_t)P2 - 1u;
     extern __shared__ uint64_t lds64[];
-    uint64_t* base = lds64;                       // [P2] reserved global start per sub-bucket
-    uint32_t* bkey = (uint32_t*)(base + P2);      // [BUF] buffered survivors
-    uint32_t* bval = bkey + BUF;                  // [BUF]
-    uint32_t* cnt = bval + BUF;                   // [P2]
-    uint32_t* wcur = cnt + P2;                    // [P2]
-    __shared__ uint32_t s_buf;
+    uint64_t* dst0 = lds64;                       // [P2] sub-bucket start + reserved base
+    uint32_t* skey = (uint32_t*)(dst0 + P2);      // [T]
+    uint32_t* sval = skey + T;                    // [T]
+    uint32_t* cnt = sval + T;                     // [P2]
+    uint32_t* off = cnt + P2;                     // [P2]
+    uint32_t* wtmp = off + P2;                    // [BLOCK/64 + 1]
+    __shared__ uint32_t s_total;
 
     const int xcd = blockIdx.x % NXCD;
     const int W = gridDim.x / NXCD;
     const int j = blockIdx.x / NXCD;
     if (j >= W) return;  // grid is a multiple of NXCD; uniform per workgroup
-    const uint32_t lane = lane_id();
     bool bad = false;
-    uint64_t surv_total = 0;
-
-    for (int b = threadIdx.x; b < P2; b += BLOCK) cnt[b] = 0u;
-    if (threadIdx.x == 0) s_buf = 0u;
-    __syncthreads();
-
-    // write the buffered survivors of first-pass bucket p; every thread calls it
-    auto flush = [&](int p) {
-        const uint32_t n = s_buf;
-        for (int b = threadIdx.x; b < P2; b += BLOCK) {
-            const uint64_t sb = ((uint64_t)p << bits2) + b;
-            const uint32_t c = cnt[b];
-            base[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
-            wcur[b] = 0u;
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-            const uint32_t kk = bkey[i];
-            const uint32_t b = (radix_hash(kk) >> shift) & bmask;
-            const uint64_t d = base[b] + atomicAdd(&wcur[b], 1u);
-            if (d < out_cap) {
-                okey[d] = kk;
-                oval[d] = bval[i];
-            } else {
-                bad = true;
-            }
-        }
-        surv_total += n;
-        __syncthreads();
-        for (int b = threadIdx.x; b < P2; b += BLOCK) cnt[b] = 0u;
-        if (threadIdx.x == 0) s_buf = 0u;
-        __syncthreads();
-    };
+    uint32_t surv_total = 0;
 
     for (int p = xcd; p < P1; p += NXCD) {
         const uint64_t lo = sb_start[(uint64_t)p << bits2];
@@ -688,12 +655,15 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
         const uint64_t clo = min(lo + (uint64_t)j * chunk, hi);
         const uint64_t chi = min(clo + chunk, hi);
         for (uint64_t tile = clo; tile < chi; tile += T) {
-            if (s_buf + (uint32_t)T > (uint32_t)BUF) flush(p);  // s_buf is stable here
             const int tn = (int)min((uint64_t)T, chi - tile);
-            uint32_t k[ITEMS];
+            for (int b = threadIdx.x; b < P2; b += BLOCK) cnt[b] = 0u;
+            __syncthreads();
+            uint32_t k[ITEMS], bk[ITEMS], rk[ITEMS];
+            bool pass[ITEMS];
 #pragma unroll
             for (int it = 0; it < ITEMS; it++) {
                 const int s = it * BLOCK + threadIdx.x;
+                // streamed once: non-temporal, so the filter slice keeps the L2
                 k[it] = (s < tn) ? __builtin_nontemporal_load(&ikey[tile + s]) : 0u;
             }
 #pragma unroll
@@ -704,20 +674,46 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
                     const uint64_t m = bloom_mask(k[it], g.k);
                     ok = (filter[bloom_block(k[it], g)] & m) == m;
                 }
-                const uint64_t bal = __ballot(ok);
-                uint32_t wb = 0;
-                if (lane == 0 && bal) wb = atomicAdd(&s_buf, (uint32_t)__popcll(bal));
-                wb = __shfl(wb, 0, WAVE);
+                pass[it] = ok;
                 if (ok) {
-                    const uint32_t slot = wb + lanes_below(bal);
-                    bkey[slot] = k[it];
-                    bval[slot] = __builtin_nontemporal_load(&ival[tile + s]);
-                    atomicAdd(&cnt[(radix_hash(k[it]) >> shift) & bmask], 1u);
+                    bk[it] = (radix_hash(k[it]) >> shift) & bmask;
+                    rk[it] = atomicAdd(&cnt[bk[it]], 1u);
                 }
             }
             __syncthreads();
+            const uint32_t tot = lds_exclusive_scan<BLOCK>(cnt, off, P2, wtmp);
+            if (threadIdx.x == 0) s_total = tot;
+            for (int b = threadIdx.x; b < P2; b += BLOCK) {
+                const uint64_t sb = ((uint64_t)p << bits2) + b;
+                const uint32_t c = cnt[b];
+                dst0[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++) {
+                if (pass[it]) {
+                    const int s = it * BLOCK + threadIdx.x;
+                    const uint32_t q = off[bk[it]] + rk[it];
+                    skey[q] = k[it];
+                    sval[q] = __builtin_nontemporal_load(&ival[tile + s]);
+                }
+            }
+            __syncthreads();
+            const uint32_t ts = s_total;
+            for (uint32_t q = threadIdx.x; q < ts; q += BLOCK) {
+                const uint32_t kk = skey[q];
+                const uint32_t b = (radix_hash(kk) >> shift) & bmask;
+                const uint64_t d = dst0[b] + (q - off[b]);
+                if (d < out_cap) {
+                    okey[d] = kk;
+                    oval[d] = sval[q];
+                } else {
+                    bad = true;
+                }
+            }
+            surv_total += ts;
+            __syncthreads();
         }
-        flush(p);
     }
     if (threadIdx.x == 0 && surv_total)
         atomicAdd((unsigned long long*)count, (unsigned long long)surv_total);
@@ -796,18 +792,19 @@ __global__ void k_plan_fill(const uint64_t* __restrict__ r_start, const uint64_t
 // Persistent workgroups pull work items from a global counter (the next item
 // is fetched while the current one runs): first the vlsplit items (the
 // largest), then one item per regular sub-bucket.
-// Per item the smaller side's chunk goes into an LDS chained hash table
-// (head[] holds index+1 per bucket, nxt[] links; keys only, 6 bytes per build
-// tuple, so two workgroups fit a CU).  The other side's slice then streams
-// through it twice, in rounds of BLOCK*PITEMS keys:
-//   pass 1 counts the item's matches (no barriers); one global atomic per item
-//          reserves its rows;
-//   pass 2 recounts per round, places each lane's rows after a block scan and
-//          writes them.  Its key re-reads are cache hits (the slice was just read).
-// Both values of a row are read only for matches: the build side's from its
-// (cache-resident) chunk, the probe side's from its slice.
+//
+// Per item, the smaller side's chunk is counting-sorted into an LDS table by
+// the hash bits below the radix bits: binfo[t] packs (start, count) of bucket
+// t, hkey[] holds the keys bucket by bucket and hidx[] their row in the chunk
+// (6 bytes per build tuple; two workgroups fit a CU).  A probe key reads one
+// binfo word, then scans a short contiguous run of keys: no pointer chasing.
+// The other side's slice streams through in rounds of BLOCK*PITEMS keys (the
+// next round is loaded while the current one is processed).  Each round counts
+// its matches, reserves its rows with one global atomic after a block scan,
+// then writes them.  Both values of a row are read only for matches: the build
+// side's from its (cache-resident) chunk, the probe side's from its slice.
 template <int BLOCK, int CAP, int PITEMS>
-__global__ __launch_bounds__(BLOCK) void k_probe(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_probe(
     const uint32_t* __restrict__ rk, const uint32_t* __restrict__ rv,
     const uint64_t* __restrict__ r_start, const uint32_t* __restrict__ sk,
     const uint32_t* __restrict__ sv, const uint64_t* __restrict__ s_start,
@@ -818,10 +815,11 @@ __global__ __launch_bounds__(BLOCK) void k_probe(
 {
     constexpr int NBMAX = CAP / 2;
     constexpr int ROUND = BLOCK * PITEMS;
-    constexpr int BU = 4;  // build loads in flight per thread
+    constexpr int BPT = CAP / BLOCK;  // build tuples per thread
+    static_assert(CAP % BLOCK == 0 && CAP <= 65535, "table indices are 16-bit");
     __shared__ uint32_t hkey[CAP];
-    __shared__ uint16_t nxt[CAP];
-    __shared__ uint32_t head[NBMAX];
+    __shared__ uint16_t hidx[CAP];
+    __shared__ uint32_t binfo[NBMAX];
     __shared__ uint32_t wtmp[BLOCK / WAVE + 1];
     __shared__ uint64_t s_item, s_next;
     __shared__ unsigned long long s_base;
@@ -857,7 +855,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe(
             const int bn = (int)min((uint64_t)CAP, nb - b_lo);
             const uint64_t per = (np + nsl - 1) / nsl;
             const uint64_t p_lo = min((uint64_t)sl * per, np);
-            const uint64_t pn = min(per, np - p_lo);
+            const uint32_t pn = (uint32_t)min(per, np - p_lo);
             const uint32_t* bkp = (build_r ? rk : sk) + (build_r ? r0 : s0) + b_lo;
             const uint32_t* bvp = (build_r ? rv : sv) + (build_r ? r0 : s0) + b_lo;
             const uint64_t prow0 = (build_r ? s0 : r0) + p_lo;
@@ -869,114 +867,107 @@ __global__ __launch_bounds__(BLOCK) void k_probe(
             while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= NBMAX) tb++;
             const int nbk = 1 << tb;
             const int tshift = 32 - tb;
-            for (int i = threadIdx.x; i < nbk; i += BLOCK) head[i] = 0u;
+            for (int i = threadIdx.x; i < nbk; i += BLOCK) binfo[i] = 0u;
             __syncthreads();
-            for (int i0 = threadIdx.x; i0 < bn; i0 += BU * BLOCK) {
-                uint32_t kb[BU];
+            // build: rank each tuple in its bucket, scan the counts, place
+            // bucket (< 2^12) and rank (< 2^16) of each build tuple, packed
+            uint32_t bkey[BPT], btr[BPT];
 #pragma unroll
-                for (int u = 0; u < BU; u++) {
-                    const int i = i0 + u * BLOCK;
-                    kb[u] = (i < bn) ? bkp[i] : 0u;
+            for (int u = 0; u < BPT; u++) {
+                const int i = u * BLOCK + threadIdx.x;
+                bkey[u] = (i < bn) ? bkp[i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < BPT; u++) {
+                const int i = u * BLOCK + threadIdx.x;
+                if (i < bn) {
+                    const uint32_t t = tb ? ((radix_hash(bkey[u]) << bits) >> tshift) : 0u;
+                    btr[u] = (atomicAdd(&binfo[t], 1u) << 16) | t;
                 }
+            }
+            __syncthreads();
+            {
+                // exclusive scan of the counts; binfo[t] = start | count << 16
+                const int per_t = (nbk + BLOCK - 1) / BLOCK;
+                const int lo_t = min((int)threadIdx.x * per_t, nbk), hi_t = min(lo_t + per_t, nbk);
+                uint32_t sum = 0;
+                for (int t = lo_t; t < hi_t; t++) sum += binfo[t];
+                uint32_t tot;
+                uint32_t ex = block_exclusive_scan<BLOCK, uint32_t>(sum, wtmp, &tot);
+                for (int t = lo_t; t < hi_t; t++) {
+                    const uint32_t c = binfo[t];
+                    binfo[t] = ex | (c << 16);
+                    ex += c;
+                }
+            }
+            __syncthreads();
 #pragma unroll
-                for (int u = 0; u < BU; u++) {
-                    const int i = i0 + u * BLOCK;
-                    if (i < bn) {
-                        hkey[i] = kb[u];
-                        const uint32_t t = tb ? ((radix_hash(kb[u]) << bits) >> tshift) : 0u;
-                        nxt[i] = (uint16_t)atomicExch(&head[t], (uint32_t)(i + 1));
-                    }
+            for (int u = 0; u < BPT; u++) {
+                const int i = u * BLOCK + threadIdx.x;
+                if (i < bn) {
+                    const uint32_t q = (binfo[btr[u] & 0xFFFFu] & 0xFFFFu) + (btr[u] >> 16);
+                    hkey[q] = bkey[u];
+                    hidx[q] = (uint16_t)i;
                 }
             }
             __syncthreads();
 
-            // pass 1: count the item's matches
-            uint32_t mine = 0;
-            {
-                uint32_t nk[PITEMS];
+            uint32_t nk[PITEMS];
+#pragma unroll
+            for (int it = 0; it < PITEMS; it++) {
+                const uint32_t idx = (uint32_t)it * BLOCK + threadIdx.x;
+                nk[it] = (idx < pn) ? pkp[idx] : 0u;
+            }
+            for (uint32_t base = 0; base < pn; base += ROUND) {
+                uint32_t kk[PITEMS], info[PITEMS];
+#pragma unroll
+                for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
+                // prefetch the next round
 #pragma unroll
                 for (int it = 0; it < PITEMS; it++) {
-                    const uint64_t idx = (uint64_t)it * BLOCK + threadIdx.x;
-                    nk[it] = (idx < pn) ? pkp[idx] : 0u;
+                    const uint32_t idx = base + ROUND + (uint32_t)it * BLOCK + threadIdx.x;
+                    if (idx < pn) nk[it] = pkp[idx];
                 }
-                for (uint64_t base = 0; base < pn; base += ROUND) {
-                    uint32_t kk[PITEMS];
 #pragma unroll
-                    for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
-#pragma unroll
-                    for (int it = 0; it < PITEMS; it++) {
-                        const uint64_t idx = base + ROUND + (uint64_t)it * BLOCK + threadIdx.x;
-                        if (idx < pn) nk[it] = pkp[idx];
-                    }
-#pragma unroll
-                    for (int it = 0; it < PITEMS; it++) {
-                        const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
-                        if (idx < pn) {
-                            const uint32_t k = kk[it];
-                            for (uint32_t e = head[tb ? ((radix_hash(k) << bits) >> tshift) : 0u]; e;
-                                 e = nxt[e - 1])
-                                mine += (hkey[e - 1] == k);
-                        }
-                    }
+                for (int it = 0; it < PITEMS; it++) {
+                    const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
+                    info[it] = (idx < pn) ? binfo[tb ? ((radix_hash(kk[it]) << bits) >> tshift) : 0u] : 0u;
                 }
-            }
-            uint32_t item_total;
-            block_exclusive_scan<BLOCK, uint32_t>(mine, wtmp, &item_total);
-            if (item_total) {
+                // count
+                uint32_t c = 0, hitmask = 0;
+#pragma unroll
+                for (int it = 0; it < PITEMS; it++) {
+                    const uint32_t st = info[it] & 0xFFFFu, en = st + (info[it] >> 16);
+                    uint32_t m = 0;
+                    for (uint32_t q = st; q < en; q++) m += (hkey[q] == kk[it]);
+                    c += m;
+                    hitmask |= (m ? 1u : 0u) << it;
+                }
+                uint32_t total;
+                const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, wtmp, &total);
+                if (total == 0) continue;  // uniform
                 if (threadIdx.x == 0)
-                    s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)item_total);
+                    s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
                 __syncthreads();
-                uint64_t run = s_base;
-                // pass 2: place and write, round by round
-                for (uint64_t base = 0; base < pn; base += ROUND) {
-                    uint32_t kk[PITEMS], first[PITEMS];
-                    uint32_t c = 0;
+                uint64_t w = s_base + pre;
 #pragma unroll
-                    for (int it = 0; it < PITEMS; it++) {
-                        const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
-                        kk[it] = (idx < pn) ? pkp[idx] : 0u;
-                    }
-#pragma unroll
-                    for (int it = 0; it < PITEMS; it++) {
-                        const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
-                        uint32_t f = 0;
-                        if (idx < pn) {
-                            const uint32_t k = kk[it];
-                            f = head[tb ? ((radix_hash(k) << bits) >> tshift) : 0u];
-                            for (uint32_t e = f; e; e = nxt[e - 1]) c += (hkey[e - 1] == k);
-                        }
-                        first[it] = f;
-                    }
-                    uint32_t rtotal;
-                    const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, wtmp, &rtotal);
-                    if (c) {
-                        uint64_t w = run + pre;
-#pragma unroll
-                        for (int it = 0; it < PITEMS; it++) {
-                            const uint64_t idx = base + (uint64_t)it * BLOCK + threadIdx.x;
-                            if (idx < pn && first[it]) {
-                                const uint32_t k = kk[it];
-                                uint32_t pval = 0;
-                                bool have = false;
-                                for (uint32_t e = first[it]; e; e = nxt[e - 1]) {
-                                    if (hkey[e - 1] == k) {
-                                        if (!have) {
-                                            pval = pvp[idx];
-                                            have = true;
-                                        }
-                                        if (w < out_cap) {
-                                            const uint32_t bval = bvp[e - 1];
-                                            o_key[w] = k;
-                                            o_rval[w] = build_r ? bval : pval;
-                                            o_sval[w] = build_r ? pval : bval;
-                                        }
-                                        w++;
-                                    }
+                for (int it = 0; it < PITEMS; it++) {
+                    if (hitmask & (1u << it)) {
+                        const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
+                        const uint32_t pval = pvp[idx];
+                        const uint32_t st = info[it] & 0xFFFFu, en = st + (info[it] >> 16);
+                        for (uint32_t q = st; q < en; q++) {
+                            if (hkey[q] == kk[it]) {
+                                if (w < out_cap) {
+                                    const uint32_t bval = bvp[hidx[q]];
+                                    o_key[w] = kk[it];
+                                    o_rval[w] = build_r ? bval : pval;
+                                    o_sval[w] = build_r ? pval : bval;
                                 }
+                                w++;
                             }
                         }
                     }
-                    run += rtotal;
                 }
             }
         }
@@ -1027,7 +1018,6 @@ constexpr int SC_BLOCK = 1024;
 constexpr int SC_ITEMS = 4;
 constexpr int FL_BLOCK = 512;
 constexpr int FL_ITEMS = 8;
-constexpr int FL_BUF = 8192;
 
 // Allow dynamic LDS up to the device limit minus the kernel's static LDS.
 static hipError_t set_lds(const void* fn, size_t bytes)
@@ -1046,7 +1036,7 @@ hipError_t kernels_init(int max_lds_bytes)
     if ((e = set_lds((const void*)k_hist<HIST_BLOCK>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds((const void*)k_scatter<SC_BLOCK, SC_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds((const void*)k_bloom_build<1024>, max_lds_bytes)) != hipSuccess) return e;
-    if ((e = set_lds((const void*)k_filter_l1<FL_BLOCK, FL_ITEMS, FL_BUF>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds((const void*)k_filter_l1<FL_BLOCK, FL_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -1106,10 +1096,17 @@ hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* 
 // 64 KB of LDS).
 int scatter_carry(int nbk) { return nbk <= 512 ? 16 : (nbk <= 1024 ? 8 : 4); }
 
+static int log2i(int x)
+{
+    int l = 0;
+    while ((1 << l) < x) l++;
+    return l;
+}
+
 size_t scatter_lds_bytes(int nbk)
 {
     const int cs = scatter_carry(nbk);
-    return (size_t)nbk * 8 + (size_t)nbk * cs * 8 + (size_t)SC_BLOCK * SC_ITEMS * 8 + (size_t)nbk * 12 +
+    return (size_t)nbk * 8 + (size_t)nbk * cs * 8 + (size_t)SC_BLOCK * SC_ITEMS * 8 + (size_t)nbk * 16 +
            (SC_BLOCK / WAVE + 1) * 4;
 }
 
@@ -1129,8 +1126,8 @@ hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* 
     const int nbk = (mode == 0) ? (1 << p1) : (1 << (bits - p1));
     const int grid = (mode == 0) ? G : (1 << p1) * W2;
     k_scatter<SC_BLOCK, SC_ITEMS><<<grid, SC_BLOCK, scatter_lds_bytes(nbk), st>>>(
-        ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, scatter_carry(nbk), l1off,
-        sb_start, l2cur, err);
+        ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, log2i(scatter_carry(nbk)),
+        l1off, sb_start, l2cur, err);
     return hipGetLastError();
 }
 
@@ -1160,10 +1157,12 @@ hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const ui
                             uint64_t* err, int num_cu, hipStream_t st)
 {
     const int P2 = 1 << bits2;
-    const size_t lds = (size_t)P2 * 8 + (size_t)FL_BUF * 8 + (size_t)P2 * 8;
+    const size_t lds = (size_t)P2 * 8 + (size_t)FL_BLOCK * FL_ITEMS * 8 + (size_t)P2 * 8 +
+                       (FL_BLOCK / WAVE + 1) * 4;
     const int W = std::max(1, num_cu / 8) * 2;  // workgroups per XCD
-    k_filter_l1<FL_BLOCK, FL_ITEMS, FL_BUF><<<8 * W, FL_BLOCK, lds, st>>>(
-        ikey, ival, sb_start, bits, bits2, filter, g, okey, oval, out_cap, s_fill, count, err);
+    k_filter_l1<FL_BLOCK, FL_ITEMS><<<8 * W, FL_BLOCK, lds, st>>>(ikey, ival, sb_start, bits, bits2, filter,
+                                                                    g, okey, oval, out_cap, s_fill, count,
+                                                                    err);
     return hipGetLastError();
 }
 

@@ -179,7 +179,7 @@ int  bj_join(bj_ctx *ctx, const bj_params *p,
  * bits bj_join would use for p (r_size = n).  Writes the partitioned tuples to
  * out_key/out_val (n entries each) and the sub-bucket starts to out_start
  * (2^(bits1+bits2) + 1 entries; bucket b of a key = the top bits of
- * lowbias32(key)).  Counterpart of partial_jointable_gpu_partition_r_only
+ * key * 0x9E3779B1 mod 2^32).  Counterpart of partial_jointable_gpu_partition_r_only
  * (src/vlsplit_hash_kernel.h:40-46). */
 int  bj_partition(bj_ctx *ctx, const bj_params *p, const uint32_t *key,
                   const uint32_t *val, int64_t n, uint32_t *out_key,

@@ -318,20 +318,22 @@ long long orc_nested_loop(const uint32_t *r_key, const uint32_t *r_val, long lon
 /* ------------------------------------------------------------------------- */
 /* Radix hash and blocked Bloom addressing, restated from the product spec.  */
 /* ------------------------------------------------------------------------- */
-uint32_t orc_radix_hash(uint32_t key) { return orc_lowbias32(key); }
+uint32_t orc_radix_hash(uint32_t key) { return key * 0x9E3779B1u; }
 
 static void bloom_addr(uint32_t key, int seg_bits, uint64_t wps, int k, uint64_t *block,
                        uint64_t *mask)
 {
-    const uint32_t h = orc_lowbias32(key);
+    const uint32_t h = orc_radix_hash(key);
     const uint64_t seg = seg_bits ? (uint64_t)(h >> (32 - seg_bits)) : 0;
     const uint32_t rest = seg_bits ? (h << seg_bits) : h;
     *block = seg * wps + (((uint64_t)rest * wps) >> 32);
-    const uint32_t h2 = orc_lowbias32(key ^ 0x9E3779B9u);
-    const uint32_t h3 = orc_lowbias32(h2 ^ 0x85EBCA6Bu);
+    uint32_t h2 = (key ^ 0x5BD1E995u) * 0xC2B2AE3Du;
+    h2 ^= h2 >> 15;
+    uint32_t h3 = h2 * 0x27D4EB2Fu;
+    h3 ^= h3 >> 15;
     uint64_t m = 0;
     for (int i = 0; i < k; i++) {
-        uint32_t bit = (i < 5) ? ((h2 >> (6 * i)) & 63u) : ((h3 >> (6 * (i - 5))) & 63u);
+        uint32_t bit = (i < 5) ? ((h2 >> (26 - 6 * i)) & 63u) : ((h3 >> (26 - 6 * (i - 5))) & 63u);
         m |= 1ULL << bit;
     }
     *mask = m;

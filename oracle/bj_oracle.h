@@ -17,8 +17,9 @@
  *                      the sentinel so every uint32 key is legal.
  *   orc_nested_loop    brute force |R|x|S| join for tiny inputs.
  *   orc_gen_*          restatement of the synthetic generator (integer only) that
- *                      the product generates on the device; see DESIGN.md.
- *   orc_bloom_*        restatement of the blocked Bloom filter addressing.
+ *                      the product runs on the device (k_gen_r / k_gen_s).
+ *   orc_radix_hash     the partition hash (key * 0x9E3779B1, Fibonacci hashing).
+ *   orc_bloom_*        restatement of the register-blocked Bloom filter addressing.
  *
  * Pins (tests/test_oracle.py): orc_sanity_count vs the reference's own
  * sanity_check.cpp compiled into oracle/_ref; orc_cpu_join vs orc_nested_loop on
