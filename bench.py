@@ -37,6 +37,13 @@ WORKLOADS = {
     "join640m_nobf": (640_000_000, 640_000_000, 0.10, 0.0, dict(partition_size=512, radix_bits=14)),
     "join64m_1280m_zipf": (64_000_000, 1_280_000_000, 0.01, 1.0,
                            dict(use_bf=1, bf_bits_per_key=12.0, num_hashes=4)),
+    # batched form: 256 joins, |R| log-uniform in [64K, 16M], |S| uniform in [1, 4] x |R|,
+    # match ratio uniform in [1%, 50%], join i generated with seed i
+    "batch256": (None, None, None, 0.0, dict(joins=256)),
+    # serving form: 8 resident R of 16M..320M keys (geometric), requests with a fresh S
+    # of log-uniform size in [1M, 256M], 10% match, Poisson arrivals at 70% of the
+    # single-join path's capacity
+    "serve8": (None, None, 0.10, 0.0, dict(relations=8, pool=16, load=0.7)),
 }
 
 
@@ -102,6 +109,247 @@ def cpu_baseline(n_r_full: int, n_s_full: int, ratio: float, budget_s: float = 1
                       f"after a warm-up; {med:.3f} s per join"}
 
 
+def batch_sizes(n_joins: int, seed: int = 0):
+    """Sizes and match ratios of the batched workload (deterministic)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_joins):
+        nr = int(np.exp(rng.uniform(np.log(64 * 1024), np.log(16 * 1024 * 1024))))
+        ns = int(nr * rng.uniform(1.0, 4.0))
+        ratio = float(rng.uniform(0.01, 0.50))
+        out.append((nr, ns, ratio))
+    return out
+
+
+def run_batch(args, n_joins: int) -> dict:
+    """The batched form: all joins in one graph replay vs the single-join path one at a time."""
+    import numpy as np
+    import torch
+    import brigjoin as bj
+
+    sizes = batch_sizes(n_joins)
+    tot_r = sum(s[0] for s in sizes)
+    tot_s = sum(s[1] for s in sizes)
+    dev = torch.device("cuda:0")
+    R = torch.empty((2, tot_r), dtype=torch.int32, device=dev)
+    S = torch.empty((2, tot_s), dtype=torch.int32, device=dev)
+    ctx = bj.Context(0)
+    specs, plain, expect = [], [], []
+    orr = ors = 0
+    for i, (nr, ns, ratio) in enumerate(sizes):
+        rk, rv = R[0, orr:orr + nr], R[1, orr:orr + nr]
+        sk, sv = S[0, ors:ors + ns], S[1, ors:ors + ns]
+        m = int(min(ns, max(0, round(ratio * ns))))
+        ctx.gen_build(i, nr, rk, rv)
+        ctx.gen_probe(i, i ^ 0xDEADBEEF, nr, ns, m, sk, sv)
+        specs.append(bj.join_spec(rk, rv, sk, sv, r_size=nr, s_size=ns, match_hint=ratio))
+        plain.append((bj.default_params(r_size=nr, s_size=ns, inputs_on_device=1), (rk, rv, sk, sv)))
+        expect.append(m)
+        orr += nr
+        ors += ns
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tuned = bj.autotune(ctx, specs)
+    tune_s = time.perf_counter() - t0
+    table = bj.tune_get(ctx)
+
+    batch = bj.Batch(ctx, specs, use_graph=True)
+    for _ in range(args.warmup):
+        outs, _ = batch.run()
+    torch.cuda.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        outs, ms = batch.run()
+        dev_ms.append(ms)
+    wall = time.perf_counter() - t0
+    bad = [i for i, o in enumerate(outs) if o.match_count != expect[i]]
+    if bad:
+        raise RuntimeError(f"batched join counts differ from the closed form at joins {bad[:8]}")
+    ms_step = wall / args.steps * 1e3
+    value = (tot_r + tot_s) / (wall / args.steps)
+
+    # the single-join path, one join at a time (same inputs, its own automatic plan)
+    for p, d in plain[:4]:
+        ctx.join(p, *d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_dev = 0.0
+    for i, (p, d) in enumerate(plain):
+        r = ctx.join(p, *d)
+        plain_dev += r.phase_ms[6]
+        if r.match_count != expect[i]:
+            raise RuntimeError(f"single-join count differs at join {i}")
+    torch.cuda.synchronize()
+    plain_wall = time.perf_counter() - t0
+    plain_value = (tot_r + tot_s) / plain_wall
+
+    # whole-batch algorithmic bytes (per join: the two-pass formula of its configuration)
+    alg = 0.0
+    for (nr, ns, ratio), o in zip(sizes, outs):
+        m = int(o.match_count)
+        f = int(round(nr * o.bf_bits_per_key / 8)) if o.bf_k else 0
+        alg += total_algorithmic_bytes(nr, ns, m, m, f, bool(o.bf_k))
+    dms = float(np.median(dev_ms))
+    achieved = alg / (dms * 1e-3) / 1e9
+    batch.close()
+    ctx.close()
+    return {
+        "metric": "input tuples/s at 640M⋈640M on 1 MI355X; % HBM roofline; batched vs plain",
+        "value": value, "unit": "input tuples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (device generator per join, seed = join index)",
+        "config": {"workload": "batch256", "joins": n_joins, "tuples_r": tot_r, "tuples_s": tot_s,
+                   "graph": True, "autotuned_classes": tuned},
+        "roofline": {"bound": "hbm", "kernel": "whole batch (graph replay)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "alg_bytes": alg, "avg_ms": round(dms, 4)},
+        "batched_vs_plain": {"plain_value": plain_value, "plain_ms": plain_wall * 1e3,
+                             "plain_device_ms": plain_dev, "speedup": round(value / plain_value, 3)},
+        "autotune": {"seconds": round(tune_s, 2),
+                     "table": [{"kind": e.kind, "lg": e.lg, "sr": e.sr, "sel": e.sel,
+                                "radix_bits": e.radix_bits, "tile": e.tile,
+                                "bf_bits_per_key": e.bf_bits_per_key, "ms": round(e.ms, 4)}
+                               for e in table]},
+    }
+
+
+def run_serve(args, opts: dict) -> dict:
+    """The serving form under Poisson arrivals vs the single-join path per request."""
+    import numpy as np
+    import torch
+    import brigjoin as bj
+
+    rng = np.random.default_rng(1)
+    nrel, npool, load = opts["relations"], opts["pool"], opts["load"]
+    rel_sizes = [int(round(16_000_000 * (20.0 ** (i / (nrel - 1))))) for i in range(nrel)]
+    dev = torch.device("cuda:0")
+    ctx = bj.Context(0)
+    R = []
+    for i, n in enumerate(rel_sizes):
+        rk = torch.empty(n, dtype=torch.int32, device=dev)
+        rv = torch.empty(n, dtype=torch.int32, device=dev)
+        ctx.gen_build(1000 + i, n, rk, rv)
+        R.append((rk, rv, n))
+    pool = []
+    for j in range(npool):
+        ri = int(rng.integers(0, nrel))
+        ns = int(np.exp(rng.uniform(np.log(1 << 20), np.log(256 << 20))))
+        sk = torch.empty(ns, dtype=torch.int32, device=dev)
+        sv = torch.empty(ns, dtype=torch.int32, device=dev)
+        m = int(round(0.10 * ns))
+        ctx.gen_probe(1000 + ri, 5000 + j, rel_sizes[ri], ns, m, sk, sv)
+        pool.append((ri, sk, sv, ns, m))
+    torch.cuda.synchronize()
+
+    # the single-join path per request: partitions R and S, filter from R, probe
+    plain_ms = []
+    for ri, sk, sv, ns, m in pool:
+        rk, rv, nr = R[ri]
+        p = bj.default_params(r_size=nr, s_size=ns, inputs_on_device=1, use_bf=1, bf_bits_per_key=10.0,
+                              num_hashes=3)
+        ctx.join(p, rk, rv, sk, sv)  # warm
+        t0 = time.perf_counter()
+        res = ctx.join(p, rk, rv, sk, sv)
+        plain_ms.append((time.perf_counter() - t0) * 1e3)
+        if res.match_count != m:
+            raise RuntimeError("single-join count differs from the closed form")
+    ctx.close()
+    mean_plain = float(np.mean(plain_ms))
+    lam = load / (mean_plain * 1e-3)
+
+    srv = bj.Server(0, slots=opts.get("slots", 3))
+    t0 = time.perf_counter()
+    for rk, rv, n in R:
+        srv.add_relation(rk, rv, n)
+    torch.cuda.synchronize()
+    reg_s = time.perf_counter() - t0
+    alone_ms = []
+    for ri, sk, sv, ns, m in pool:
+        for _ in range(2):
+            rid = srv.submit(ri, sk, sv, ns)
+            r = srv.wait(rid)
+            srv.release(rid)
+        if r.status != 0 or r.match_count != m:
+            raise RuntimeError("served count differs from the closed form")
+        alone_ms.append((r.t_end - r.t_submit) * 1e3)
+
+    # Poisson arrivals over the run; completed requests are polled and released
+    duration = args.serve_seconds
+    arrivals = []
+    t = 0.0
+    while True:
+        t += float(rng.exponential(1.0 / lam))
+        if t > duration:
+            break
+        arrivals.append((t, int(rng.integers(0, npool))))
+    lat, tuples, done = [], 0, 0
+    bad = 0
+    submit_at = {}
+    start = time.perf_counter()
+    k = 0
+    last_progress = start
+    while k < len(arrivals) or done < len(arrivals):
+        now = time.perf_counter() - start
+        while k < len(arrivals) and arrivals[k][0] <= now:
+            ri, sk, sv, ns, m = pool[arrivals[k][1]]
+            rid = srv.submit(ri, sk, sv, ns)
+            submit_at[rid] = (arrivals[k][1], m)
+            k += 1
+        for r in srv.poll(512):
+            j, m = submit_at.pop(r.id)
+            if r.status != 0 or r.match_count != m:
+                bad += 1
+            lat.append((r.t_end - r.t_submit) * 1e3)
+            tuples += pool[j][3] + rel_sizes[pool[j][0]]
+            done += 1
+            srv.release(r.id)
+        if time.perf_counter() - last_progress > 10.0:
+            print(f"[serve] t={now:.1f}s submitted={k} done={done}", file=sys.stderr, flush=True)
+            last_progress = time.perf_counter()
+        nxt = arrivals[k][0] - (time.perf_counter() - start) if k < len(arrivals) else 0.0005
+        time.sleep(max(0.0, min(nxt, 0.0005)))
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    if bad:
+        raise RuntimeError(f"{bad} served requests had a wrong count")
+
+    # the single-join path under the same arrivals: one FIFO server with the measured
+    # per-request times (Lindley recursion)
+    free_at, plain_lat = 0.0, []
+    for t_arr, j in arrivals:
+        begin = max(t_arr, free_at)
+        free_at = begin + plain_ms[j] * 1e-3
+        plain_lat.append((free_at - t_arr) * 1e3)
+    tot_plain_tuples = sum(pool[j][3] + rel_sizes[pool[j][0]] for _, j in arrivals)
+    lat = np.array(lat)
+    value = tuples / wall
+    return {
+        "metric": "input tuples/s at 640M⋈640M on 1 MI355X; % HBM roofline; batched vs plain",
+        "value": value, "unit": "input tuples/s", "n_gpus": 1, "steps": len(arrivals),
+        "warmup": len(pool), "ms_per_step": float(np.mean(lat)), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (device generator; S pool reused across requests)",
+        "config": {"workload": "serve8", "relations": rel_sizes, "pool_s_sizes": [p[3] for p in pool],
+                   "arrival_rate_per_s": round(lam, 2), "offered_load_vs_plain": load,
+                   "seconds": duration, "slots": opts.get("slots", 3)},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "mean": round(float(lat.mean()), 3), "max": round(float(lat.max()), 3)},
+        "served": {"requests": int(len(lat)), "tuples_per_s": value, "wall_s": round(wall, 2),
+                   "max_in_flight": int(st.max_in_flight), "arena_peak_bytes": int(st.arena_peak_bytes),
+                   "relation_bytes": int(st.relation_bytes), "register_s": round(reg_s, 3),
+                   "alone_ms_mean": round(float(np.mean(alone_ms)), 3)},
+        "plain_per_request": {"service_ms_mean": round(mean_plain, 3),
+                              "capacity_tuples_per_s": tot_plain_tuples / (sum(plain_ms[j] for _, j in arrivals) * 1e-3),
+                              "p50_ms_same_arrivals": round(float(np.percentile(plain_lat, 50)), 3),
+                              "p99_ms_same_arrivals": round(float(np.percentile(plain_lat, 99)), 3)},
+    }
+
+
 def load_traffic(workload: str, kernel: str):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -118,6 +366,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="join640m_bf10", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve8: length of the arrival process")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--bf-stage", type=int, default=None, help="override: 1 before, 2 after pass 1")
     ap.add_argument("--pass1-bits", type=int, default=None, help="override the first scatter pass bits")
@@ -145,6 +394,12 @@ def main() -> int:
         return 0
 
     n_r, n_s, ratio, zipf_s, params = WORKLOADS[args.workload]
+    if args.workload == "batch256":
+        print(json.dumps(run_batch(args, params["joins"])), flush=True)
+        return 0
+    if args.workload == "serve8":
+        print(json.dumps(run_serve(args, params)), flush=True)
+        return 0
     m_s = int(min(n_s, max(0, round(ratio * n_s))))
     dev = torch.device("cuda:0")
     rk = torch.empty(n_r, dtype=torch.int32, device=dev)

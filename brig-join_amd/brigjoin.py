@@ -29,6 +29,12 @@ EXPORTS = [
     "bj_last_error", "bj_version", "bj_join", "bj_copy_rows_to_host", "bj_gen_build",
     "bj_gen_probe", "bj_write_output", "bj_read_output", "bj_free_host",
     "bj_cpu_sanity_count", "bj_main_args_default", "bj_gpu_main", "bj_plan", "bj_partition",
+    "bj_join_spec_default", "bj_batch_plan", "bj_batch_create", "bj_batch_run", "bj_batch_destroy",
+    "bj_autotune", "bj_tune_get", "bj_tune_set",
+    "bj_server_opts_default", "bj_server_create", "bj_server_add_relation", "bj_server_request_bytes",
+    "bj_server_submit", "bj_server_poll", "bj_server_wait", "bj_server_copy_rows", "bj_server_release",
+    "bj_server_drain",
+    "bj_server_stats_get", "bj_server_destroy",
 ]
 
 
@@ -96,6 +102,64 @@ class BjMainArgs(ctypes.Structure):
     ]
 
 
+class BjJoinSpec(ctypes.Structure):
+    _fields_ = [
+        ("r_key", ctypes.c_void_p), ("r_val", ctypes.c_void_p),
+        ("s_key", ctypes.c_void_p), ("s_val", ctypes.c_void_p),
+        ("r_size", ctypes.c_int64), ("s_size", ctypes.c_int64),
+        ("radix_bits", ctypes.c_int32), ("tile", ctypes.c_int32),
+        ("bf_bits_per_key", ctypes.c_double), ("num_hashes", ctypes.c_int32),
+        ("answer_scale", ctypes.c_double), ("match_hint", ctypes.c_double),
+    ]
+
+
+class BjJoinOut(ctypes.Structure):
+    _fields_ = [
+        ("match_count", ctypes.c_uint64), ("rows_written", ctypes.c_uint64),
+        ("capacity", ctypes.c_uint64), ("s_survivors", ctypes.c_uint64),
+        ("truncated", ctypes.c_int32), ("radix_bits", ctypes.c_int32),
+        ("tile", ctypes.c_int32), ("bf_k", ctypes.c_int32),
+        ("bf_bits_per_key", ctypes.c_double),
+        ("key", ctypes.c_void_p), ("r_val", ctypes.c_void_p), ("s_val", ctypes.c_void_p),
+    ]
+
+
+class BjTuneEntry(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("lg", ctypes.c_int32), ("sr", ctypes.c_int32),
+        ("sel", ctypes.c_int32), ("radix_bits", ctypes.c_int32), ("tile", ctypes.c_int32),
+        ("bf_bits_per_key", ctypes.c_double), ("ms", ctypes.c_double),
+    ]
+
+
+class BjServerOpts(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_int32), ("arena_bytes", ctypes.c_uint64), ("use_bf", ctypes.c_int32),
+        ("bf_bits_per_key", ctypes.c_double), ("num_hashes", ctypes.c_int32),
+        ("radix_bits", ctypes.c_int32), ("answer_scale", ctypes.c_double),
+    ]
+
+
+class BjRequestResult(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("relation", ctypes.c_int32),
+        ("s_size", ctypes.c_uint64), ("match_count", ctypes.c_uint64),
+        ("rows_written", ctypes.c_uint64), ("capacity", ctypes.c_uint64),
+        ("s_survivors", ctypes.c_uint64), ("truncated", ctypes.c_int32),
+        ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+        ("key", ctypes.c_void_p), ("r_val", ctypes.c_void_p), ("s_val", ctypes.c_void_p),
+    ]
+
+
+class BjServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("released", ctypes.c_uint64),
+        ("max_in_flight", ctypes.c_int32), ("arena_bytes", ctypes.c_uint64),
+        ("arena_peak_bytes", ctypes.c_uint64), ("relation_bytes", ctypes.c_uint64),
+    ]
+
+
 _lib = None
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
@@ -145,6 +209,35 @@ def lib():
         L.bj_gpu_main.argtypes = [ctypes.POINTER(BjMainArgs), ctypes.POINTER(BjResult)]
         L.bj_plan.argtypes = [ctypes.POINTER(BjParams), ctypes.POINTER(BjPlanInfo)]
         L.bj_partition.argtypes = [vp, ctypes.POINTER(BjParams), vp, vp, ctypes.c_int64, vp, vp, vp]
+        L.bj_join_spec_default.argtypes = [ctypes.POINTER(BjJoinSpec)]
+        L.bj_join_spec_default.restype = None
+        L.bj_batch_plan.argtypes = [vp, ctypes.c_int, ctypes.POINTER(BjJoinSpec),
+                                    ctypes.POINTER(BjJoinOut), ctypes.POINTER(ctypes.c_uint64)]
+        L.bj_batch_create.argtypes = [vp, ctypes.c_int, ctypes.POINTER(BjJoinSpec), ctypes.c_int,
+                                      ctypes.POINTER(vp)]
+        L.bj_batch_run.argtypes = [vp, ctypes.POINTER(BjJoinOut), ctypes.POINTER(ctypes.c_double)]
+        L.bj_batch_destroy.argtypes = [vp]
+        L.bj_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(BjJoinSpec),
+                                  ctypes.POINTER(ctypes.c_int)]
+        L.bj_tune_get.argtypes = [vp, ctypes.POINTER(BjTuneEntry), ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int)]
+        L.bj_tune_set.argtypes = [vp, ctypes.POINTER(BjTuneEntry), ctypes.c_int]
+        L.bj_server_opts_default.argtypes = [ctypes.POINTER(BjServerOpts)]
+        L.bj_server_opts_default.restype = None
+        L.bj_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(BjServerOpts), ctypes.POINTER(vp)]
+        L.bj_server_add_relation.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]
+        L.bj_server_request_bytes.argtypes = [vp, ctypes.c_int32, ctypes.c_int64,
+                                              ctypes.POINTER(ctypes.c_uint64)]
+        L.bj_server_submit.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.c_int64,
+                                       ctypes.POINTER(ctypes.c_uint64)]
+        L.bj_server_poll.argtypes = [vp, ctypes.POINTER(BjRequestResult), ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.bj_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(BjRequestResult)]
+        L.bj_server_copy_rows.argtypes = [vp, ctypes.POINTER(BjRequestResult), vp, vp, vp, ctypes.c_uint64]
+        L.bj_server_release.argtypes = [vp, ctypes.c_uint64]
+        L.bj_server_drain.argtypes = [vp]
+        L.bj_server_stats_get.argtypes = [vp, ctypes.POINTER(BjServerStats)]
+        L.bj_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -238,6 +331,183 @@ class Context:
                   zipf_s: float = 0.0) -> None:
         check(lib().bj_gen_probe(self.h, seed_r, seed_s, n_r, n_s, m_s, zipf_s, _ptr(d_key),
                                  _ptr(d_val)), "bj_gen_probe")
+
+
+def join_spec(r_key=None, r_val=None, s_key=None, s_val=None, r_size=0, s_size=0, **kw) -> BjJoinSpec:
+    s = BjJoinSpec()
+    lib().bj_join_spec_default(ctypes.byref(s))
+    s.r_key, s.r_val, s.s_key, s.s_val = (_ptr(x) for x in (r_key, r_val, s_key, s_val))
+    s.r_size, s.s_size = r_size, s_size
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        setattr(s, k, v)
+    return s
+
+
+def _spec_array(specs):
+    arr = (BjJoinSpec * max(len(specs), 1))()
+    for i, s in enumerate(specs):
+        arr[i] = s
+    return arr
+
+
+def batch_plan(specs, ctx: "Context | None" = None):
+    """Configuration and device footprint of a batch (no device needed without ctx)."""
+    arr = _spec_array(specs)
+    outs = (BjJoinOut * max(len(specs), 1))()
+    ws = ctypes.c_uint64()
+    check(lib().bj_batch_plan(ctx.h if ctx else None, len(specs), arr, outs, ctypes.byref(ws)),
+          "bj_batch_plan")
+    return [outs[i] for i in range(len(specs))], int(ws.value)
+
+
+class Batch:
+    """Many joins in one launch sequence (captured in a hipGraph when use_graph)."""
+
+    def __init__(self, ctx: Context, specs, use_graph: bool = True):
+        self.ctx = ctx
+        self.n = len(specs)
+        self._specs = _spec_array(specs)
+        self.h = ctypes.c_void_p()
+        check(lib().bj_batch_create(ctx.h, self.n, self._specs, 1 if use_graph else 0,
+                                    ctypes.byref(self.h)), "bj_batch_create")
+
+    def run(self):
+        outs = (BjJoinOut * max(self.n, 1))()
+        ms = ctypes.c_double()
+        check(lib().bj_batch_run(self.h, outs, ctypes.byref(ms)), "bj_batch_run")
+        return [outs[i] for i in range(self.n)], float(ms.value)
+
+    def rows_to_host(self, out: BjJoinOut):
+        """Rows of one join's result (copied through a BjResult view)."""
+        r = BjResult()
+        r.rows_written = out.rows_written
+        r.key, r.r_val, r.s_val = out.key, out.r_val, out.s_val
+        return self.ctx.rows_to_host(r)
+
+    def close(self):
+        if self.h:
+            lib().bj_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(ctx: Context, specs) -> int:
+    arr = _spec_array(specs)
+    n = ctypes.c_int()
+    check(lib().bj_autotune(ctx.h, len(specs), arr, ctypes.byref(n)), "bj_autotune")
+    return n.value
+
+
+def tune_get(ctx: Context):
+    n = ctypes.c_int()
+    check(lib().bj_tune_get(ctx.h, None, 0, ctypes.byref(n)), "bj_tune_get")
+    arr = (BjTuneEntry * max(n.value, 1))()
+    check(lib().bj_tune_get(ctx.h, arr, n.value, ctypes.byref(n)), "bj_tune_get")
+    return [arr[i] for i in range(n.value)]
+
+
+def tune_set(ctx: Context, entries) -> None:
+    arr = (BjTuneEntry * max(len(entries), 1))()
+    for i, e in enumerate(entries):
+        arr[i] = e
+    check(lib().bj_tune_set(ctx.h, arr, len(entries)), "bj_tune_set")
+
+
+def server_opts(**kw) -> BjServerOpts:
+    o = BjServerOpts()
+    lib().bj_server_opts_default(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Server:
+    """Request loop over device-resident relations (continuous batching)."""
+
+    def __init__(self, device: int = 0, **opts):
+        self.h = ctypes.c_void_p()
+        o = server_opts(**opts)
+        check(lib().bj_server_create(device, ctypes.byref(o), ctypes.byref(self.h)), "bj_server_create")
+        self._ctx = None
+
+    def add_relation(self, r_key, r_val, n_r: int) -> int:
+        rid = ctypes.c_int32()
+        check(lib().bj_server_add_relation(self.h, _ptr(r_key), _ptr(r_val), n_r, ctypes.byref(rid)),
+              "bj_server_add_relation")
+        return rid.value
+
+    def request_bytes(self, rel: int, n_s: int) -> int:
+        b = ctypes.c_uint64()
+        check(lib().bj_server_request_bytes(self.h, rel, n_s, ctypes.byref(b)), "bj_server_request_bytes")
+        return b.value
+
+    def submit(self, rel: int, s_key, s_val, n_s: int) -> int:
+        rid = ctypes.c_uint64()
+        check(lib().bj_server_submit(self.h, rel, _ptr(s_key), _ptr(s_val), n_s, ctypes.byref(rid)),
+              "bj_server_submit")
+        return rid.value
+
+    def poll(self, cap: int = 256):
+        arr = (BjRequestResult * cap)()
+        n = ctypes.c_int()
+        check(lib().bj_server_poll(self.h, arr, cap, ctypes.byref(n)), "bj_server_poll")
+        return [arr[i] for i in range(n.value)]
+
+    def wait(self, rid: int) -> BjRequestResult:
+        r = BjRequestResult()
+        check(lib().bj_server_wait(self.h, rid, ctypes.byref(r)), "bj_server_wait")
+        return r
+
+    def release(self, rid: int) -> None:
+        check(lib().bj_server_release(self.h, rid), "bj_server_release")
+
+    def drain(self) -> None:
+        check(lib().bj_server_drain(self.h), "bj_server_drain")
+
+    def stats(self) -> BjServerStats:
+        st = BjServerStats()
+        check(lib().bj_server_stats_get(self.h, ctypes.byref(st)), "bj_server_stats_get")
+        return st
+
+    def rows_to_host(self, r: BjRequestResult):
+        """Rows of a completed request, copied to the host (before release)."""
+        n = int(r.rows_written)
+        k, a, b = (np.empty(max(n, 1), np.uint32) for _ in range(3))
+        check(lib().bj_server_copy_rows(self.h, ctypes.byref(r), k.ctypes.data, a.ctypes.data,
+                                        b.ctypes.data, n), "bj_server_copy_rows")
+        return k[:n], a[:n], b[:n]
+
+    def close(self):
+        if self.h:
+            lib().bj_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def write_output(path: str, key, r_val, s_val) -> None:

@@ -12,6 +12,8 @@
 // size the device produces (|S'|, work items) is read by the next kernel from
 // device memory, and the host reads the counters once at the end.
 #include "bj_api.h"
+#include "bj_batch.h"
+#include "bj_internal.h"
 #include "bj_kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -21,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 
 using namespace bj;
@@ -29,30 +32,17 @@ namespace {
 
 thread_local std::string g_last_error;
 
+}  // namespace
+
+namespace bj {
 int fail(int code, const std::string& msg)
 {
     g_last_error = msg;
     return code;
 }
+}  // namespace bj
 
-#define BJ_HIP(expr)                                                                         \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(BJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
-enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_N = 8 };
-
-// Kernel slot k (BJ_K_*) is bracketed by events EV_KB + k and EV_KE + k.
-enum { EV_H2D0 = 0, EV_H2D1, EV_KB, EV_KE = EV_KB + BJ_NUM_KERNELS, EV_N = EV_KE + BJ_NUM_KERNELS };
-
-uint64_t splitmix64(uint64_t x)
+uint64_t bj::splitmix64(uint64_t x)
 {
     uint64_t z = x + 0x9E3779B97F4A7C15ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -60,7 +50,7 @@ uint64_t splitmix64(uint64_t x)
     return z ^ (z >> 31);
 }
 
-PermKeys perm_keys(uint64_t seed)
+PermKeys bj::perm_keys(uint64_t seed)
 {
     PermKeys K;
     uint64_t st = seed;
@@ -73,37 +63,20 @@ PermKeys perm_keys(uint64_t seed)
     return K;
 }
 
-int ceil_log2(uint64_t n)
+int bj::ceil_log2(uint64_t n)
 {
     int l = 0;
     while ((1ull << l) < n) l++;
     return l;
 }
 
+namespace {
+
 bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
 
 }  // namespace
 
-struct bj_ctx {
-    int device = 0;
-    int num_cu = 256;
-    int max_lds = 64 * 1024;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[EV_N] = {};
-    // inputs copied from the host (only when inputs_on_device == 0)
-    DevBuf in_rk, in_rv, in_sk, in_sv;
-    // partitioned relations and scratch
-    DevBuf r_key, r_val, s_key, s_val, tmp_key, tmp_val, sp_key, sp_val;
-    DevBuf filter, part, totals, l1h, l1off, l2cur, r_start, s_start, s_fill, partials, items, counters;
-    DevBuf plan_cnt, item_start;
-    DevBuf out_key, out_rval, out_sval, dtmp;
-    uint64_t* h_counters = nullptr;  // pinned
-};
-
-namespace {
-
-int ensure(DevBuf& b, size_t bytes)
+int bj::ensure(DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (b.bytes >= bytes) return BJ_OK;
@@ -124,7 +97,7 @@ int ensure(DevBuf& b, size_t bytes)
     return BJ_OK;
 }
 
-void release(DevBuf& b)
+void bj::release(DevBuf& b)
 {
     if (b.p) {
         hipError_t e = hipFree(b.p);
@@ -134,26 +107,7 @@ void release(DevBuf& b)
     b.bytes = 0;
 }
 
-template <typename T>
-T* P(DevBuf& b) { return (T*)b.p; }
-
-// ----------------------------------------------------------------- the plan
-struct Plan {
-    uint64_t nr = 0, ns = 0;
-    int bits = 1, bits1 = 1, bits2 = 0;  // logical levels (partition_size, second level)
-    int p1 = 1;                          // bits of the first physical scatter pass
-    int W2 = 1;                          // second-pass workgroups per first-pass bucket
-    int G_r = 1, G_s = 1;
-    int tile = 8192;
-    uint32_t slice = 65536;
-    uint32_t max_slices = 16;
-    bool bf = false;
-    int bf_stage = 0;
-    BloomGeom geom{0, 8, 1};
-    uint64_t bf_words = 0;
-    uint64_t cap = 1;
-    uint64_t item_cap = 1;
-};
+namespace bj {
 
 constexpr uint64_t MAX_TUPLES = 0x7FFFFFFFull;  // row ids are 31-bit in the probe's staging
 constexpr int MAX_BITS = 15;                     // joint histogram in LDS: 2^15 x 4 B
@@ -350,48 +304,58 @@ int run_partition(bj_ctx* c, const Plan& pl, const uint32_t* key, const uint32_t
     return BJ_OK;
 }
 
-// Allocates (or, with c == nullptr, only sums) the device workspace of a join.
-int ensure_workspace(bj_ctx* c, const Plan& pl, uint64_t* total_bytes = nullptr)
+int workspace_items(const Plan& pl, bool with_r, bool with_out,
+                    const std::function<int(DevBuf bj_ctx::*, uint64_t)>& f)
 {
-    uint64_t total = 0;
-    auto need = [&](DevBuf* b, uint64_t bytes) -> int {
-        bytes = std::max<uint64_t>(bytes, 256);
-        total += bytes;
-        return c ? ensure(*b, bytes) : BJ_OK;
-    };
-    auto B = [&](DevBuf bj_ctx::*m) -> DevBuf* { return c ? &(c->*m) : nullptr; };
     int rc;
+    auto need = [&](DevBuf bj_ctx::*m, uint64_t bytes) { return f(m, std::max<uint64_t>(bytes, 256)); };
     const uint64_t nr = std::max<uint64_t>(pl.nr, 1), ns = std::max<uint64_t>(pl.ns, 1);
     const uint64_t nbins = 1ull << pl.bits;
-    const uint64_t gmax = (uint64_t)std::max(pl.G_r, pl.G_s);
+    const uint64_t gmax = (uint64_t)(with_r ? std::max(pl.G_r, pl.G_s) : pl.G_s);
     const uint64_t n_l1 = (1ull << pl.p1) * gmax;
-    if ((rc = need(B(&bj_ctx::r_key), nr * 4)) || (rc = need(B(&bj_ctx::r_val), nr * 4))) return rc;
-    if ((rc = need(B(&bj_ctx::s_key), ns * 4)) || (rc = need(B(&bj_ctx::s_val), ns * 4))) return rc;
+    if (with_r) {
+        if ((rc = need(&bj_ctx::r_key, nr * 4)) || (rc = need(&bj_ctx::r_val, nr * 4))) return rc;
+        if ((rc = need(&bj_ctx::r_start, (nbins + 1) * 8))) return rc;
+        if (pl.bf && (rc = need(&bj_ctx::filter, pl.bf_words * 4))) return rc;
+    }
+    if ((rc = need(&bj_ctx::s_key, ns * 4)) || (rc = need(&bj_ctx::s_val, ns * 4))) return rc;
     if (pl.p1 < pl.bits || (pl.bf && pl.bf_stage == 2)) {
-        const uint64_t t = std::max(nr, ns);
-        if ((rc = need(B(&bj_ctx::tmp_key), t * 4)) || (rc = need(B(&bj_ctx::tmp_val), t * 4))) return rc;
+        const uint64_t t = with_r ? std::max(nr, ns) : ns;
+        if ((rc = need(&bj_ctx::tmp_key, t * 4)) || (rc = need(&bj_ctx::tmp_val, t * 4))) return rc;
     }
     if (pl.bf) {
         if (pl.bf_stage == 1) {
-            if ((rc = need(B(&bj_ctx::sp_key), ns * 4)) || (rc = need(B(&bj_ctx::sp_val), ns * 4))) return rc;
+            if ((rc = need(&bj_ctx::sp_key, ns * 4)) || (rc = need(&bj_ctx::sp_val, ns * 4))) return rc;
         }
-        if ((rc = need(B(&bj_ctx::filter), pl.bf_words * 4))) return rc;
-        if ((rc = need(B(&bj_ctx::s_fill), nbins * 4))) return rc;
+        if ((rc = need(&bj_ctx::s_fill, nbins * 4))) return rc;
     }
-    if ((rc = need(B(&bj_ctx::part), gmax * nbins * 4))) return rc;
-    if ((rc = need(B(&bj_ctx::totals), nbins * 4))) return rc;
-    if ((rc = need(B(&bj_ctx::l1h), n_l1 * 4))) return rc;
-    if ((rc = need(B(&bj_ctx::l1off), n_l1 * 8))) return rc;
-    if ((rc = need(B(&bj_ctx::l2cur), (uint64_t)pl.W2 * nbins * 8))) return rc;
-    if ((rc = need(B(&bj_ctx::plan_cnt), nbins * 4))) return rc;
-    if ((rc = need(B(&bj_ctx::item_start), (nbins + 1) * 8))) return rc;
-    if ((rc = need(B(&bj_ctx::r_start), (nbins + 1) * 8)) || (rc = need(B(&bj_ctx::s_start), (nbins + 1) * 8)))
-        return rc;
-    if ((rc = need(B(&bj_ctx::partials), scan_partials_needed(nbins) * 8 + 64))) return rc;
-    if ((rc = need(B(&bj_ctx::items), pl.item_cap * sizeof(HeavyItem)))) return rc;
-    if ((rc = need(B(&bj_ctx::out_key), pl.cap * 4)) || (rc = need(B(&bj_ctx::out_rval), pl.cap * 4)) ||
-        (rc = need(B(&bj_ctx::out_sval), pl.cap * 4)))
-        return rc;
+    if ((rc = need(&bj_ctx::part, gmax * nbins * 4))) return rc;
+    if ((rc = need(&bj_ctx::totals, nbins * 4))) return rc;
+    if ((rc = need(&bj_ctx::l1h, n_l1 * 4))) return rc;
+    if ((rc = need(&bj_ctx::l1off, n_l1 * 8))) return rc;
+    if ((rc = need(&bj_ctx::l2cur, (uint64_t)pl.W2 * nbins * 8))) return rc;
+    if ((rc = need(&bj_ctx::plan_cnt, nbins * 4))) return rc;
+    if ((rc = need(&bj_ctx::item_start, (nbins + 1) * 8))) return rc;
+    if ((rc = need(&bj_ctx::s_start, (nbins + 1) * 8))) return rc;
+    if ((rc = need(&bj_ctx::partials, scan_partials_needed(nbins) * 8 + 64))) return rc;
+    if ((rc = need(&bj_ctx::items, pl.item_cap * sizeof(HeavyItem)))) return rc;
+    if ((rc = need(&bj_ctx::counters, CTR_N * sizeof(uint64_t)))) return rc;
+    if (with_out) {
+        if ((rc = need(&bj_ctx::out_key, pl.cap * 4)) || (rc = need(&bj_ctx::out_rval, pl.cap * 4)) ||
+            (rc = need(&bj_ctx::out_sval, pl.cap * 4)))
+            return rc;
+    }
+    return BJ_OK;
+}
+
+int ensure_workspace(bj_ctx* c, const Plan& pl, uint64_t* total_bytes)
+{
+    uint64_t total = 0;
+    int rc = workspace_items(pl, true, true, [&](DevBuf bj_ctx::*m, uint64_t bytes) -> int {
+        total += bytes;
+        return c ? ensure(c->*m, bytes) : BJ_OK;
+    });
+    if (rc) return rc;
     if (total_bytes) *total_bytes = total;
     return BJ_OK;
 }
@@ -406,7 +370,81 @@ float ev_ms(bj_ctx* c, int a, int b)
     return ms;
 }
 
-}  // namespace
+int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* r_val)
+{
+    int rc;
+    if ((rc = run_partition(c, pl, r_key, r_val, pl.nr, nullptr, pl.G_r, P<uint32_t>(c->r_key),
+                            P<uint32_t>(c->r_val), c->r_key.bytes / 4, P<uint64_t>(c->r_start), BJ_K_HIST_R)))
+        return rc;
+    if (pl.bf) {
+        BJ_HIP(hipEventRecord(c->ev[EV_KB + BJ_K_BF_BUILD], c->stream));
+        BJ_HIP(launch_bloom_build(P<uint32_t>(c->r_key), P<uint64_t>(c->r_start), pl.bits, pl.geom,
+                                  P<uint64_t>(c->filter), c->stream));
+        BJ_HIP(hipEventRecord(c->ev[EV_KE + BJ_K_BF_BUILD], c->stream));
+    }
+    return BJ_OK;
+}
+
+int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
+                    uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, uint64_t cap)
+{
+    hipStream_t st = c->stream;
+    uint64_t* ctr = P<uint64_t>(c->counters);
+    auto beg = [&](int slot) { return hipEventRecord(c->ev[EV_KB + slot], st); };
+    auto end = [&](int slot) { return hipEventRecord(c->ev[EV_KE + slot], st); };
+    int rc;
+    const uint32_t* s_cnt = nullptr;
+    if (pl.bf) {
+        if (pl.bf_stage == 1) {
+            // the reference's order: filter S, then partition the survivors
+            BJ_HIP(beg(BJ_K_BF_FILTER));
+            BJ_HIP(launch_bloom_prefilter(s_key, s_val, pl.ns, R.filter, pl.geom, P<uint32_t>(c->sp_key),
+                                          P<uint32_t>(c->sp_val), c->sp_key.bytes / 4, ctr + CTR_SPRIME,
+                                          ctr + CTR_ERR, c->num_cu, st));
+            BJ_HIP(end(BJ_K_BF_FILTER));
+            if ((rc = run_partition(c, pl, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns, ctr + CTR_SPRIME,
+                                    pl.G_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), c->s_key.bytes / 4,
+                                    P<uint64_t>(c->s_start), BJ_K_HIST_S)))
+                return rc;
+        } else {
+            // first-level pass of S, then filter each bucket against its L2-resident
+            // filter slice while placing the survivors into their sub-buckets
+            if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->tmp_key),
+                                    P<uint32_t>(c->tmp_val), c->tmp_key.bytes / 4, P<uint64_t>(c->s_start),
+                                    BJ_K_HIST_S, true)))
+                return rc;
+            BJ_HIP(beg(BJ_K_BF_FILTER));
+            BJ_HIP(launch_zero_u32(P<uint32_t>(c->s_fill), 1ull << pl.bits, st));
+            BJ_HIP(launch_filter_l1(P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val), P<uint64_t>(c->s_start),
+                                    pl.bits, pl.bits - pl.p1, R.filter, pl.geom, P<uint32_t>(c->s_key),
+                                    P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint32_t>(c->s_fill),
+                                    ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
+            s_cnt = P<uint32_t>(c->s_fill);
+            BJ_HIP(end(BJ_K_BF_FILTER));
+        }
+    } else {
+        if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->s_key),
+                                P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint64_t>(c->s_start),
+                                BJ_K_HIST_S)))
+            return rc;
+    }
+    const int nbins = 1 << pl.bits;
+    uint64_t* item_start = P<uint64_t>(c->item_start);
+    BJ_HIP(beg(BJ_K_PLAN));
+    BJ_HIP(launch_plan(R.start, P<uint64_t>(c->s_start), s_cnt, pl.bits, (uint32_t)pl.tile, pl.slice,
+                       pl.max_slices, P<uint32_t>(c->plan_cnt), item_start, P<uint64_t>(c->partials),
+                       P<HeavyItem>(c->items), pl.item_cap, ctr + CTR_ERR, st));
+    BJ_HIP(end(BJ_K_PLAN));
+    BJ_HIP(beg(BJ_K_PROBE));
+    BJ_HIP(launch_probe(pl.tile, R.key, R.val, R.start, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val),
+                        P<uint64_t>(c->s_start), s_cnt, pl.bits, pl.slice, P<HeavyItem>(c->items),
+                        item_start + nbins, pl.item_cap, ctr + CTR_WORK, ctr + CTR_OUT, cap, o_key, o_rval,
+                        o_sval, c->num_cu, st));
+    BJ_HIP(end(BJ_K_PROBE));
+    return BJ_OK;
+}
+
+}  // namespace bj
 
 // ===================================================================== C ABI
 extern "C" {
@@ -473,6 +511,7 @@ int bj_create(bj_ctx** out, int device)
     c->max_lds = max_lds;
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = kernels_init(max_lds);
+    if (e == hipSuccess) e = batch_kernels_init(max_lds);
     for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_counters, (CTR_N + 1) * sizeof(uint64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -583,70 +622,17 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
         return e == hipSuccess ? end(slot) : e;
     };
 
-    // R: two-level partition
-    if ((rc = run_partition(c, pl, r_key, r_val, pl.nr, nullptr, pl.G_r, P<uint32_t>(c->r_key),
-                            P<uint32_t>(c->r_val), c->r_key.bytes / 4, P<uint64_t>(c->r_start),
-                            BJ_K_HIST_R)))
-        return rc;
-
-    const uint32_t* s_cnt = nullptr;
-    if (pl.bf) {
-        BJ_HIP(beg(BJ_K_BF_BUILD));
-        BJ_HIP(launch_bloom_build(P<uint32_t>(c->r_key), P<uint64_t>(c->r_start), pl.bits, pl.geom,
-                                  P<uint64_t>(c->filter), st));
-        BJ_HIP(end(BJ_K_BF_BUILD));
-        if (pl.bf_stage == 1) {
-            // the reference's order: filter S, then partition the survivors
-            BJ_HIP(beg(BJ_K_BF_FILTER));
-            BJ_HIP(launch_bloom_prefilter(s_key, s_val, pl.ns, P<uint64_t>(c->filter), pl.geom,
-                                          P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val),
-                                          c->sp_key.bytes / 4, ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
-            BJ_HIP(end(BJ_K_BF_FILTER));
-            if ((rc = run_partition(c, pl, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns,
-                                    ctr + CTR_SPRIME, pl.G_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val),
-                                    c->s_key.bytes / 4, P<uint64_t>(c->s_start), BJ_K_HIST_S)))
-                return rc;
-        } else {
-            // first-level pass of S, then filter each bucket against its L2-resident
-            // filter slice while placing the survivors into their sub-buckets
-            if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->tmp_key),
-                                    P<uint32_t>(c->tmp_val), c->tmp_key.bytes / 4, P<uint64_t>(c->s_start),
-                                    BJ_K_HIST_S, true)))
-                return rc;
-            BJ_HIP(beg(BJ_K_BF_FILTER));
-            BJ_HIP(launch_zero_u32(P<uint32_t>(c->s_fill), 1ull << pl.bits, st));
-            BJ_HIP(launch_filter_l1(P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val), P<uint64_t>(c->s_start),
-                                    pl.bits, pl.bits - pl.p1, P<uint64_t>(c->filter), pl.geom,
-                                    P<uint32_t>(c->s_key),
-                                    P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint32_t>(c->s_fill),
-                                    ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
-            s_cnt = P<uint32_t>(c->s_fill);
-            BJ_HIP(end(BJ_K_BF_FILTER));
-        }
-    } else {
+    if (!pl.bf) {
         BJ_HIP(skip(BJ_K_BF_BUILD));
         BJ_HIP(skip(BJ_K_BF_FILTER));
-        if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->s_key),
-                                P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint64_t>(c->s_start),
-                                BJ_K_HIST_S)))
-            return rc;
     }
-
+    if ((rc = enqueue_r(c, pl, r_key, r_val))) return rc;
+    const RSide R{P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start), P<uint64_t>(c->filter)};
+    if ((rc = enqueue_s_probe(c, pl, R, s_key, s_val, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
+                              P<uint32_t>(c->out_sval), pl.cap)))
+        return rc;
     const int nbins = 1 << pl.bits;
     uint64_t* item_start = P<uint64_t>(c->item_start);
-    BJ_HIP(beg(BJ_K_PLAN));
-    BJ_HIP(launch_plan(P<uint64_t>(c->r_start), P<uint64_t>(c->s_start), s_cnt, pl.bits, (uint32_t)pl.tile,
-                       pl.slice, pl.max_slices, P<uint32_t>(c->plan_cnt), item_start, P<uint64_t>(c->partials),
-                       P<HeavyItem>(c->items), pl.item_cap, ctr + CTR_ERR, st));
-    BJ_HIP(end(BJ_K_PLAN));
-    BJ_HIP(beg(BJ_K_PROBE));
-    BJ_HIP(launch_probe(pl.tile, P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start),
-                        P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), P<uint64_t>(c->s_start), s_cnt, pl.bits,
-                        pl.slice, P<HeavyItem>(c->items), item_start + nbins, pl.item_cap, ctr + CTR_WORK,
-                        ctr + CTR_OUT,
-                        pl.cap, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
-                        P<uint32_t>(c->out_sval), c->num_cu, st));
-    BJ_HIP(end(BJ_K_PROBE));
 
     BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     BJ_HIP(hipMemcpyAsync(c->h_counters + CTR_N, item_start + nbins, sizeof(uint64_t), hipMemcpyDeviceToHost, st));

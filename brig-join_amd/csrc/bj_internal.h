@@ -1,0 +1,114 @@
+// bj_internal.h -- host-side state shared by the single-join API (bj_api.cpp)
+// and the batched join + autotuner (bj_batch.cpp).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "bj_api.h"
+#include "bj_device.h"
+
+namespace bj {
+
+// Records the detail for bj_last_error and returns code.
+int fail(int code, const std::string& msg);
+
+#define BJ_HIP(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return ::bj::fail(BJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// Grows b to at least bytes (contents are not kept).
+int ensure(DevBuf& b, size_t bytes);
+void release(DevBuf& b);
+
+template <typename T>
+T* P(DevBuf& b) { return (T*)b.p; }
+
+uint64_t splitmix64(uint64_t x);
+PermKeys perm_keys(uint64_t seed);
+int ceil_log2(uint64_t n);
+
+enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_N = 8 };
+
+// Kernel slot k (BJ_K_*) is bracketed by events EV_KB + k and EV_KE + k.
+enum { EV_H2D0 = 0, EV_H2D1, EV_KB, EV_KE = EV_KB + BJ_NUM_KERNELS, EV_N = EV_KE + BJ_NUM_KERNELS };
+
+// The plan of one join (bj_params -> radix bits, filter geometry, sizes).
+struct Plan {
+    uint64_t nr = 0, ns = 0;
+    int bits = 1, bits1 = 1, bits2 = 0;  // logical levels (partition_size, second level)
+    int p1 = 1;                          // bits of the first physical scatter pass
+    int W2 = 1;                          // second-pass workgroups per first-pass bucket
+    int G_r = 1, G_s = 1;
+    int tile = 8192;
+    uint32_t slice = 65536;
+    uint32_t max_slices = 16;
+    bool bf = false;
+    int bf_stage = 0;
+    BloomGeom geom{0, 8, 1};
+    uint64_t bf_words = 0;
+    uint64_t cap = 1;
+    uint64_t item_cap = 1;
+};
+
+// A partitioned R: keys / values in sub-bucket order, sub-bucket starts
+// (2^bits + 1 entries) and the filter built from it (null without one).
+struct RSide {
+    const uint32_t* key;
+    const uint32_t* val;
+    const uint64_t* start;
+    const uint64_t* filter;
+};
+
+int make_plan(const bj_params& p, Plan& pl);
+
+}  // namespace bj
+
+struct bj_ctx {
+    int device = 0;
+    int num_cu = 256;
+    int max_lds = 64 * 1024;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[bj::EV_N] = {};
+    // inputs copied from the host (only when inputs_on_device == 0)
+    bj::DevBuf in_rk, in_rv, in_sk, in_sv;
+    // partitioned relations and scratch
+    bj::DevBuf r_key, r_val, s_key, s_val, tmp_key, tmp_val, sp_key, sp_val;
+    bj::DevBuf filter, part, totals, l1h, l1off, l2cur, r_start, s_start, s_fill, partials, items, counters;
+    bj::DevBuf plan_cnt, item_start;
+    bj::DevBuf out_key, out_rval, out_sval, dtmp;
+    uint64_t* h_counters = nullptr;  // pinned
+    // autotuner table of the batched join (bj_autotune / bj_tune_set)
+    std::vector<bj_tune_entry> tune;
+};
+
+namespace bj {
+
+// The device buffers of a join: f(member, bytes) for each.  with_r: the R side
+// (partitioned R, its sub-bucket starts, the filter); with_out: the row buffers.
+int workspace_items(const Plan& pl, bool with_r, bool with_out,
+                    const std::function<int(DevBuf bj_ctx::*, uint64_t)>& f);
+// Allocates (or, with c == nullptr, only sums) the device workspace of a join.
+int ensure_workspace(bj_ctx* c, const Plan& pl, uint64_t* total_bytes = nullptr);
+// R partition (and the filter built from it) into c's r_key / r_val / r_start / filter.
+int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* r_val);
+// S (filtered against R's filter when the plan has one) partitioned with R's
+// radix bits, then the build + probe against R; rows to o_* (capacity cap).
+int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
+                    uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, uint64_t cap);
+float ev_ms(bj_ctx* c, int a, int b);
+
+}  // namespace bj

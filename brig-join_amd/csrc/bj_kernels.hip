@@ -21,6 +21,7 @@
 //                                    local part (src/kernel_functions.cu:294-457, 612-789)
 #include "bj_kernels.h"
 #include "bj_device.h"
+#include "bj_tiles.h"
 
 #include <math.h>
 
@@ -134,17 +135,7 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const uint32_t* __restrict__ key
     const uint64_t chunk = (n + (uint64_t)G - 1) / (uint64_t)G;
     const uint64_t lo = min((uint64_t)blockIdx.x * chunk, n);
     const uint64_t hi = min(lo + chunk, n);
-    const int sh = 32 - bits;
-    constexpr int U = 8;
-    uint64_t i = lo + threadIdx.x;
-    for (; i + (uint64_t)(U - 1) * BLOCK < hi; i += (uint64_t)U * BLOCK) {
-        uint32_t k[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) k[u] = key[i + (uint64_t)u * BLOCK];
-#pragma unroll
-        for (int u = 0; u < U; u++) atomicAdd(&h[radix_hash(k[u]) >> sh], 1u);
-    }
-    for (; i < hi; i += BLOCK) atomicAdd(&h[radix_hash(key[i]) >> sh], 1u);
+    hist_range<BLOCK>(key, lo, hi, 32 - bits, h);
     __syncthreads();
     uint32_t* dst = part + (size_t)blockIdx.x * nbins;
     for (int b = threadIdx.x; b < nbins; b += BLOCK) dst[b] = h[b];
@@ -290,16 +281,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t* __rest
 }
 
 // ============================================================== scatter
-// LDS-staged scatter with write-combining carries.
-//
-// Each tile of T tuples is counting-sorted by bucket in LDS (rank from an LDS
-// atomic, offsets from an LDS scan).  Bucket b's tuples go to consecutive global
-// positions starting at cur[b].  Only whole CS-tuple chunks (CS * 4 bytes) are
-// written; the unaligned tail of every bucket (< CS tuples) is carried in LDS
-// to the next tile, so every chunk of the output is written by one workgroup
-// within one tile.  Partial chunks occur only where two workgroups' ranges of
-// a bucket meet.  The next tile is loaded into registers while the current one
-// is written out.
+// Write-combining scatter of one range (scatter_range, bj_tiles.h).  What a
+// workgroup owns:
 //   mode 0 (pass 1, p1 bits): workgroup g owns the same chunk as k_hist's group
 //       g; bucket = h >> (32 - p1); cur[b] = l1off[g * nbk + b].
 //   mode 1 (pass 2, bits - p1 bits): workgroup (b1, w) owns the pass-1 output of
@@ -314,24 +297,12 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
     int cs_log2, const uint64_t* __restrict__ l1off, const uint64_t* __restrict__ sb_start,
     const uint64_t* __restrict__ l2cur, uint64_t* __restrict__ err)
 {
-    constexpr int T = BLOCK * ITEMS;
     const int b2 = bits - p1;
     const int nbk = (mode == 0) ? (1 << p1) : (1 << b2);
     const int shift = (mode == 0) ? (32 - p1) : (32 - bits);
     const uint32_t bmask = (uint32_t)nbk - 1u;
-    const uint32_t cs = 1u << cs_log2;
-
     extern __shared__ uint64_t lds64[];
-    uint64_t* cur = lds64;                                 // [nbk] next global position
-    uint32_t* ck = (uint32_t*)(cur + nbk);                 // [nbk * cs] carried keys
-    uint32_t* cv = ck + nbk * cs;                          // [nbk * cs] carried values
-    uint32_t* skey = cv + nbk * cs;                        // [T]
-    uint32_t* sval = skey + T;                             // [T]
-    uint32_t* cnt = sval + T;                              // [nbk] tile tuples
-    uint32_t* off = cnt + nbk;                             // [nbk] staging offset
-    uint32_t* ccnt = off + nbk;                            // [nbk] carried tuples
-    uint32_t* nw = ccnt + nbk;                             // [nbk] tuples written this tile
-    uint32_t* wtmp = nw + nbk;                             // [BLOCK/64 + 1]
+    const ScatterLds L = scatter_lds<BLOCK, ITEMS>(lds64, nbk, cs_log2);
 
     uint64_t lo, hi;
     if (mode == 0) {
@@ -339,7 +310,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
         const uint64_t chunk = (n + (uint64_t)G - 1) / (uint64_t)G;
         lo = min((uint64_t)blockIdx.x * chunk, n);
         hi = min(lo + chunk, n);
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) cur[b] = l1off[(size_t)blockIdx.x * nbk + b];
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) L.cur[b] = l1off[(size_t)blockIdx.x * nbk + b];
     } else {
         const int P1 = 1 << p1;
         const int b1 = blockIdx.x / W2, w = blockIdx.x % W2;
@@ -348,124 +319,11 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(
         hi = (g1 < G) ? l1off[(size_t)g1 * P1 + b1] : sb_start[(uint64_t)(b1 + 1) << b2];
         const uint64_t nbins = 1ull << bits;
         for (int b = threadIdx.x; b < nbk; b += BLOCK)
-            cur[b] = l2cur[(size_t)w * nbins + ((uint64_t)b1 << b2) + b];
+            L.cur[b] = l2cur[(size_t)w * nbins + ((uint64_t)b1 << b2) + b];
     }
-    for (int b = threadIdx.x; b < nbk; b += BLOCK) ccnt[b] = 0u;
-    // cur/ccnt are read by other threads; an empty range skips the tile loop's barriers
-    __syncthreads();
-
-    const uint32_t* kin = ikey + lo;
-    const uint32_t* vin = ival + lo;
-    const uint32_t len = (uint32_t)(hi - lo);  // ranges are below 2^31 tuples
-    uint32_t k[ITEMS], v[ITEMS];
-#pragma unroll
-    for (int it = 0; it < ITEMS; it++) {
-        const uint32_t i = (uint32_t)it * BLOCK + threadIdx.x;
-        if (i < len) {
-            k[it] = __builtin_nontemporal_load(&kin[i]);
-            v[it] = __builtin_nontemporal_load(&vin[i]);
-        }
-    }
-    bool bad = false;
-
-    for (uint32_t tile = 0; tile < len; tile += T) {
-        const int tn = (int)min((uint32_t)T, len - tile);
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) cnt[b] = 0u;
-        __syncthreads();
-        uint32_t bk[ITEMS], rk[ITEMS];
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const int s = it * BLOCK + threadIdx.x;
-            if (s < tn) {
-                bk[it] = (radix_hash(k[it]) >> shift) & bmask;
-                rk[it] = atomicAdd(&cnt[bk[it]], 1u);
-            }
-        }
-        __syncthreads();
-        lds_exclusive_scan<BLOCK>(cnt, off, nbk, wtmp);
-        // tuples of bucket b written this tile: up to its last whole chunk
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
-            const uint32_t a0 = (uint32_t)cur[b] & (cs - 1u);
-            const uint32_t e = a0 + ccnt[b] + cnt[b];
-            nw[b] = e >= cs ? (e & ~(cs - 1u)) - a0 : 0u;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const int s = it * BLOCK + threadIdx.x;
-            if (s < tn) {
-                const uint32_t p = off[bk[it]] + rk[it];
-                skey[p] = k[it];
-                sval[p] = v[it];
-            }
-        }
-        // prefetch the next tile while this one is written out
-        const uint32_t next = tile + T;
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const uint32_t i = next + (uint32_t)it * BLOCK + threadIdx.x;
-            if (i < len) {
-                k[it] = __builtin_nontemporal_load(&kin[i]);
-                v[it] = __builtin_nontemporal_load(&vin[i]);
-            }
-        }
-        __syncthreads();
-        // phase A: carried tuples inside the written range go out
-        for (int idx = threadIdx.x; idx < (nbk << cs_log2); idx += BLOCK) {
-            const int b = idx >> cs_log2;
-            const uint32_t i = (uint32_t)idx & (cs - 1u);
-            if (i < ccnt[b] && i < nw[b]) {
-                const uint64_t g = cur[b] + i;
-                if (g < out_cap) {
-                    okey[g] = ck[idx];
-                    oval[g] = cv[idx];
-                } else {
-                    bad = true;
-                }
-            }
-        }
-        __syncthreads();
-        // phase B: the tile's tuples go out, or into the carry
-        for (int s = threadIdx.x; s < tn; s += BLOCK) {
-            const uint32_t kk = skey[s];
-            const uint32_t b = (radix_hash(kk) >> shift) & bmask;
-            const uint32_t rel = ccnt[b] + (uint32_t)s - off[b];
-            const uint32_t w_b = nw[b];
-            if (rel < w_b) {
-                const uint64_t g = cur[b] + rel;
-                if (g < out_cap) {
-                    okey[g] = kk;
-                    oval[g] = sval[s];
-                } else {
-                    bad = true;
-                }
-            } else {
-                const uint32_t slot = (b << cs_log2) + rel - w_b;
-                ck[slot] = kk;
-                cv[slot] = sval[s];
-            }
-        }
-        __syncthreads();
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
-            cur[b] += nw[b];
-            ccnt[b] = ccnt[b] + cnt[b] - nw[b];
-        }
-        __syncthreads();
-    }
-    // the remaining carries (partial chunks at the range ends)
-    for (int idx = threadIdx.x; idx < (nbk << cs_log2); idx += BLOCK) {
-        const int b = idx >> cs_log2;
-        const uint32_t i = (uint32_t)idx & (cs - 1u);
-        if (i < ccnt[b]) {
-            const uint64_t g = cur[b] + i;
-            if (g < out_cap) {
-                okey[g] = ck[idx];
-                oval[g] = cv[idx];
-            } else {
-                bad = true;
-            }
-        }
-    }
+    // ranges are below 2^31 tuples
+    const bool bad = scatter_range<BLOCK, ITEMS>(L, ikey + lo, ival + lo, (uint32_t)(hi - lo), nbk, shift,
+                                                 bmask, cs_log2, okey, oval, out_cap);
     if (bad) atomicOr((unsigned long long*)err, 1ull);
 }
 
@@ -506,22 +364,7 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_build(const uint32_t* __restric
     const int d = bits - g.seg_bits;
     const uint64_t lo = r_start[s << d], hi = r_start[(s + 1) << d];
     const uint64_t segbase = s * (uint64_t)wps;
-    constexpr int U = 8;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += (uint64_t)U * BLOCK) {
-        uint32_t kk[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint64_t j = i + (uint64_t)u * BLOCK;
-            kk[u] = (j < hi) ? __builtin_nontemporal_load(&rkey[j]) : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            if (i + (uint64_t)u * BLOCK < hi) {
-                const uint32_t w = (uint32_t)(bloom_block(kk[u], g) - segbase);
-                atomicOr(&seg[w], (unsigned long long)bloom_mask(kk[u], g.k));
-            }
-        }
-    }
+    bloom_build_range<BLOCK>(rkey, lo, hi, g, segbase, seg);
     __syncthreads();
     uint64_t* dst = filter + segbase;
     for (uint32_t i = threadIdx.x; i < wps; i += BLOCK) dst[i] = seg[i];
@@ -530,11 +373,7 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_build(const uint32_t* __restric
 // ============================================================== Bloom filter S -> S' (before partitioning)
 // The reference's order (queryAndCompactKernelPartitioned): S is filtered
 // before it is partitioned.  Used when the filter fits the MALL, since each key
-// probes a random 256-bit block.
-// Query each S key's 256-bit block (two 16-byte loads), keep the survivors.
-// Compaction: wave ballot + mbcnt inside each (item, wave) group, an LDS scan
-// over the groups, one global atomic per tile.  s_val is read only for
-// survivors.
+// probes a random filter block.  Grid-stride over tiles (prefilter_tile).
 template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_bloom_prefilter(
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval, uint64_t n,
@@ -542,71 +381,14 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_prefilter(
     uint32_t* __restrict__ oval, uint64_t out_cap, uint64_t* __restrict__ count,
     uint64_t* __restrict__ err)
 {
-    constexpr int NW = BLOCK / WAVE;
-    constexpr int NG = ITEMS * NW;  // groups per tile
     constexpr int T = BLOCK * ITEMS;
-    static_assert(NG <= 2 * WAVE, "group scan handles at most 128 groups");
-    __shared__ uint32_t gcnt[NG];
+    __shared__ uint32_t gcnt[ITEMS * (BLOCK / WAVE)];
     __shared__ uint64_t s_base;
-    const int wid = threadIdx.x / WAVE;
-    const uint32_t lane = lane_id();
-
-    for (uint64_t tile = (uint64_t)blockIdx.x * T; tile < n; tile += (uint64_t)gridDim.x * T) {
-        uint32_t k[ITEMS];
-        bool pass[ITEMS];
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
-            k[it] = (i < n) ? __builtin_nontemporal_load(&skey[i]) : 0u;
-        }
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
-            bool ok = false;
-            if (i < n) {
-                const uint64_t m = bloom_mask(k[it], g.k);
-                ok = (filter[bloom_block(k[it], g)] & m) == m;
-            }
-            pass[it] = ok;
-        }
-        uint32_t rank[ITEMS];
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const uint64_t m = __ballot(pass[it]);
-            rank[it] = lanes_below(m);
-            if (lane == 0) gcnt[it * NW + wid] = (uint32_t)__popcll(m);
-        }
-        __syncthreads();
-        if (wid == 0) {
-            // exclusive scan of NG (<= 128) group counts: two entries per lane
-            const uint32_t a0 = (2 * lane < (uint32_t)NG) ? gcnt[2 * lane] : 0u;
-            const uint32_t a1 = (2 * lane + 1 < (uint32_t)NG) ? gcnt[2 * lane + 1] : 0u;
-            const uint32_t inc = wave_inclusive_scan<uint32_t>(a0 + a1);
-            const uint32_t ex = inc - a0 - a1;
-            if (2 * lane < (uint32_t)NG) gcnt[2 * lane] = ex;
-            if (2 * lane + 1 < (uint32_t)NG) gcnt[2 * lane + 1] = ex + a0;
-            const uint32_t total = __shfl(inc, WAVE - 1, WAVE);
-            if (lane == 0) s_base = total ? atomicAdd((unsigned long long*)count,
-                                                      (unsigned long long)total) : 0ull;
-        }
-        __syncthreads();
-        bool bad = false;
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            if (pass[it]) {
-                const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
-                const uint64_t pos = s_base + gcnt[it * NW + wid] + rank[it];
-                if (pos < out_cap) {
-                    okey[pos] = k[it];
-                    oval[pos] = sval[i];
-                } else {
-                    bad = true;
-                }
-            }
-        }
-        if (bad) atomicOr((unsigned long long*)err, 2ull);
-        __syncthreads();
-    }
+    bool bad = false;
+    for (uint64_t tile = (uint64_t)blockIdx.x * T; tile < n; tile += (uint64_t)gridDim.x * T)
+        bad |= prefilter_tile<BLOCK, ITEMS>(skey, sval, tile, n, filter, g, okey, oval, out_cap, count, gcnt,
+                                            &s_base);
+    if (bad) atomicOr((unsigned long long*)err, 2ull);
 }
 
 // ============================================================== Bloom filter after pass 1
@@ -737,18 +519,6 @@ __device__ __forceinline__ void sb_sizes(const uint64_t* r_start, const uint64_t
     ns = s_cnt ? (uint64_t)s_cnt[sb] : (s_start[sb + 1] - s0);
 }
 
-__device__ __forceinline__ uint32_t plan_items(uint64_t nr, uint64_t ns, uint32_t tile, uint32_t slice,
-                                               uint32_t max_slices, uint32_t& chunks, uint32_t& slices)
-{
-    chunks = slices = 0;
-    if (nr == 0 || ns == 0) return 0;
-    const uint64_t nb = min(nr, ns), np = max(nr, ns);
-    if (nb <= tile && np <= slice) return 0;
-    chunks = (uint32_t)((nb + tile - 1) / tile);
-    slices = (uint32_t)min<uint64_t>((np + slice - 1) / slice, max_slices);
-    return chunks * slices;
-}
-
 __global__ void k_plan_count(const uint64_t* __restrict__ r_start, const uint64_t* __restrict__ s_start,
                              const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile,
                              uint32_t slice, uint32_t max_slices, uint32_t* __restrict__ n_items)
@@ -791,18 +561,9 @@ __global__ void k_plan_fill(const uint64_t* __restrict__ r_start, const uint64_t
 // ============================================================== build + probe
 // Persistent workgroups pull work items from a global counter (the next item
 // is fetched while the current one runs): first the vlsplit items (the
-// largest), then one item per regular sub-bucket.
-//
-// Per item, the smaller side's chunk is counting-sorted into an LDS table by
-// the hash bits below the radix bits: binfo[t] packs (start, count) of bucket
-// t, hkey[] holds the keys bucket by bucket and hidx[] their row in the chunk
-// (6 bytes per build tuple; two workgroups fit a CU).  A probe key reads one
-// binfo word, then scans a short contiguous run of keys: no pointer chasing.
-// The other side's slice streams through in rounds of BLOCK*PITEMS keys (the
-// next round is loaded while the current one is processed).  Each round counts
-// its matches, reserves its rows with one global atomic after a block scan,
-// then writes them.  Both values of a row are read only for matches: the build
-// side's from its (cache-resident) chunk, the probe side's from its slice.
+// largest), then one item per regular sub-bucket.  Each item builds the
+// smaller side's chunk into an LDS table and streams the other side's slice
+// past it (probe_item, bj_tiles.h).
 template <int BLOCK, int CAP, int PITEMS>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_probe(
     const uint32_t* __restrict__ rk, const uint32_t* __restrict__ rv,
@@ -813,16 +574,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     uint64_t* __restrict__ work_ctr, uint64_t* __restrict__ out_count, uint64_t out_cap,
     uint32_t* __restrict__ o_key, uint32_t* __restrict__ o_rval, uint32_t* __restrict__ o_sval)
 {
-    constexpr int NBMAX = CAP / 2;
-    constexpr int ROUND = BLOCK * PITEMS;
-    constexpr int BPT = CAP / BLOCK;  // build tuples per thread
-    static_assert(CAP % BLOCK == 0 && CAP <= 65535, "table indices are 16-bit");
-    __shared__ uint32_t hkey[CAP];
-    __shared__ uint16_t hidx[CAP];
-    __shared__ uint32_t binfo[NBMAX];
-    __shared__ uint32_t wtmp[BLOCK / WAVE + 1];
+    __shared__ ProbeLds<BLOCK, CAP> L;
     __shared__ uint64_t s_item, s_next;
-    __shared__ unsigned long long s_base;
 
     const int nbins = 1 << bits;
     const uint64_t n_heavy = min(*n_items_p, item_cap);
@@ -856,120 +609,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             const uint64_t per = (np + nsl - 1) / nsl;
             const uint64_t p_lo = min((uint64_t)sl * per, np);
             const uint32_t pn = (uint32_t)min(per, np - p_lo);
-            const uint32_t* bkp = (build_r ? rk : sk) + (build_r ? r0 : s0) + b_lo;
-            const uint32_t* bvp = (build_r ? rv : sv) + (build_r ? r0 : s0) + b_lo;
+            const uint64_t brow0 = (build_r ? r0 : s0) + b_lo;
             const uint64_t prow0 = (build_r ? s0 : r0) + p_lo;
-            const uint32_t* pkp = (build_r ? sk : rk) + prow0;
-            const uint32_t* pvp = (build_r ? sv : rv) + prow0;
-
-            // table geometry: 2^tb buckets, about 1..2 build tuples per bucket
-            int tb = 0;
-            while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= NBMAX) tb++;
-            const int nbk = 1 << tb;
-            const int tshift = 32 - tb;
-            for (int i = threadIdx.x; i < nbk; i += BLOCK) binfo[i] = 0u;
-            __syncthreads();
-            // build: rank each tuple in its bucket, scan the counts, place
-            // bucket (< 2^12) and rank (< 2^16) of each build tuple, packed
-            uint32_t bkey[BPT], btr[BPT];
-#pragma unroll
-            for (int u = 0; u < BPT; u++) {
-                const int i = u * BLOCK + threadIdx.x;
-                bkey[u] = (i < bn) ? bkp[i] : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < BPT; u++) {
-                const int i = u * BLOCK + threadIdx.x;
-                if (i < bn) {
-                    const uint32_t t = tb ? ((radix_hash(bkey[u]) << bits) >> tshift) : 0u;
-                    btr[u] = (atomicAdd(&binfo[t], 1u) << 16) | t;
-                }
-            }
-            __syncthreads();
-            {
-                // exclusive scan of the counts; binfo[t] = start | count << 16
-                const int per_t = (nbk + BLOCK - 1) / BLOCK;
-                const int lo_t = min((int)threadIdx.x * per_t, nbk), hi_t = min(lo_t + per_t, nbk);
-                uint32_t sum = 0;
-                for (int t = lo_t; t < hi_t; t++) sum += binfo[t];
-                uint32_t tot;
-                uint32_t ex = block_exclusive_scan<BLOCK, uint32_t>(sum, wtmp, &tot);
-                for (int t = lo_t; t < hi_t; t++) {
-                    const uint32_t c = binfo[t];
-                    binfo[t] = ex | (c << 16);
-                    ex += c;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < BPT; u++) {
-                const int i = u * BLOCK + threadIdx.x;
-                if (i < bn) {
-                    const uint32_t q = (binfo[btr[u] & 0xFFFFu] & 0xFFFFu) + (btr[u] >> 16);
-                    hkey[q] = bkey[u];
-                    hidx[q] = (uint16_t)i;
-                }
-            }
-            __syncthreads();
-
-            uint32_t nk[PITEMS];
-#pragma unroll
-            for (int it = 0; it < PITEMS; it++) {
-                const uint32_t idx = (uint32_t)it * BLOCK + threadIdx.x;
-                nk[it] = (idx < pn) ? pkp[idx] : 0u;
-            }
-            for (uint32_t base = 0; base < pn; base += ROUND) {
-                uint32_t kk[PITEMS], info[PITEMS];
-#pragma unroll
-                for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
-                // prefetch the next round
-#pragma unroll
-                for (int it = 0; it < PITEMS; it++) {
-                    const uint32_t idx = base + ROUND + (uint32_t)it * BLOCK + threadIdx.x;
-                    if (idx < pn) nk[it] = pkp[idx];
-                }
-#pragma unroll
-                for (int it = 0; it < PITEMS; it++) {
-                    const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
-                    info[it] = (idx < pn) ? binfo[tb ? ((radix_hash(kk[it]) << bits) >> tshift) : 0u] : 0u;
-                }
-                // count
-                uint32_t c = 0, hitmask = 0;
-#pragma unroll
-                for (int it = 0; it < PITEMS; it++) {
-                    const uint32_t st = info[it] & 0xFFFFu, en = st + (info[it] >> 16);
-                    uint32_t m = 0;
-                    for (uint32_t q = st; q < en; q++) m += (hkey[q] == kk[it]);
-                    c += m;
-                    hitmask |= (m ? 1u : 0u) << it;
-                }
-                uint32_t total;
-                const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, wtmp, &total);
-                if (total == 0) continue;  // uniform
-                if (threadIdx.x == 0)
-                    s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
-                __syncthreads();
-                uint64_t w = s_base + pre;
-#pragma unroll
-                for (int it = 0; it < PITEMS; it++) {
-                    if (hitmask & (1u << it)) {
-                        const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
-                        const uint32_t pval = pvp[idx];
-                        const uint32_t st = info[it] & 0xFFFFu, en = st + (info[it] >> 16);
-                        for (uint32_t q = st; q < en; q++) {
-                            if (hkey[q] == kk[it]) {
-                                if (w < out_cap) {
-                                    const uint32_t bval = bvp[hidx[q]];
-                                    o_key[w] = kk[it];
-                                    o_rval[w] = build_r ? bval : pval;
-                                    o_sval[w] = build_r ? pval : bval;
-                                }
-                                w++;
-                            }
-                        }
-                    }
-                }
-            }
+            probe_item<BLOCK, CAP, PITEMS>(L, (build_r ? rk : sk) + brow0, (build_r ? rv : sv) + brow0, bn,
+                                           (build_r ? sk : rk) + prow0, (build_r ? sv : rv) + prow0, pn,
+                                           build_r, bits, out_count, out_cap, o_key, o_rval, o_sval);
         }
         __syncthreads();  // the table is rebuilt by the next item; s_next is written
         item = s_next;

@@ -1,0 +1,468 @@
+// bj_tiles.h -- device bodies shared by the single-join kernels (bj_kernels.hip)
+// and the batched kernels (bj_batch.hip).
+//
+// Each body works on one contiguous range (a histogram chunk, a scatter range,
+// a filter segment, a probe work item); the kernels around them only decide
+// which range a workgroup owns and where its results go.
+#pragma once
+
+#include "bj_device.h"
+
+namespace bj {
+
+// ---------------------------------------------------------------- histogram
+// h[] (LDS, zeroed by the caller) += histogram of the top (32 - sh) bits of the
+// radix hash over key[lo, hi).  8 loads in flight per thread.
+template <int BLOCK>
+__device__ __forceinline__ void hist_range(const uint32_t* __restrict__ key, uint64_t lo, uint64_t hi,
+                                           int sh, uint32_t* h)
+{
+    constexpr int U = 8;
+    uint64_t i = lo + threadIdx.x;
+    for (; i + (uint64_t)(U - 1) * BLOCK < hi; i += (uint64_t)U * BLOCK) {
+        uint32_t k[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) k[u] = key[i + (uint64_t)u * BLOCK];
+#pragma unroll
+        for (int u = 0; u < U; u++) atomicAdd(&h[radix_hash(k[u]) >> sh], 1u);
+    }
+    for (; i < hi; i += BLOCK) atomicAdd(&h[radix_hash(key[i]) >> sh], 1u);
+}
+
+// ---------------------------------------------------------------- scatter
+// LDS-staged scatter with write-combining carries.
+//
+// Each tile of T tuples is counting-sorted by bucket in LDS (rank from an LDS
+// atomic, offsets from an LDS scan).  Bucket b's tuples go to consecutive global
+// positions starting at cur[b].  Only whole CS-tuple chunks (CS * 4 bytes) are
+// written; the unaligned tail of every bucket (< CS tuples) is carried in LDS
+// to the next tile, so every chunk of the output is written by one workgroup
+// within one tile.  Partial chunks occur only where two workgroups' ranges of
+// a bucket meet.  The next tile is loaded into registers while the current one
+// is written out.
+struct ScatterLds {
+    uint64_t* cur;   // [nbk] next global position
+    uint32_t* ck;    // [nbk * cs] carried keys
+    uint32_t* cv;    // [nbk * cs] carried values
+    uint32_t* skey;  // [T]
+    uint32_t* sval;  // [T]
+    uint32_t* cnt;   // [nbk] tile tuples
+    uint32_t* off;   // [nbk] staging offset
+    uint32_t* ccnt;  // [nbk] carried tuples
+    uint32_t* nw;    // [nbk] tuples written this tile
+    uint32_t* wtmp;  // [BLOCK/64 + 1]
+};
+
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ ScatterLds scatter_lds(uint64_t* lds64, int nbk, int cs_log2)
+{
+    constexpr int T = BLOCK * ITEMS;
+    ScatterLds L;
+    L.cur = lds64;
+    L.ck = (uint32_t*)(L.cur + nbk);
+    L.cv = L.ck + (nbk << cs_log2);
+    L.skey = L.cv + (nbk << cs_log2);
+    L.sval = L.skey + T;
+    L.cnt = L.sval + T;
+    L.off = L.cnt + nbk;
+    L.ccnt = L.off + nbk;
+    L.nw = L.ccnt + nbk;
+    L.wtmp = L.nw + nbk;
+    return L;
+}
+
+// Scatters kin/vin[0, len) by bucket (radix_hash(key) >> shift) & bmask into
+// okey/oval at the cursors L.cur[] (initialised by the caller; this function
+// begins with a barrier).  Writes are bounds-checked against out_cap; returns
+// true on a violation.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ bool scatter_range(const ScatterLds& L, const uint32_t* __restrict__ kin,
+                                              const uint32_t* __restrict__ vin, uint32_t len, int nbk,
+                                              int shift, uint32_t bmask, int cs_log2,
+                                              uint32_t* __restrict__ okey, uint32_t* __restrict__ oval,
+                                              uint64_t out_cap)
+{
+    constexpr int T = BLOCK * ITEMS;
+    const uint32_t cs = 1u << cs_log2;
+    for (int b = threadIdx.x; b < nbk; b += BLOCK) L.ccnt[b] = 0u;
+    // cur/ccnt are read by other threads; an empty range skips the tile loop's barriers
+    __syncthreads();
+
+    uint32_t k[ITEMS], v[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t i = (uint32_t)it * BLOCK + threadIdx.x;
+        if (i < len) {
+            k[it] = __builtin_nontemporal_load(&kin[i]);
+            v[it] = __builtin_nontemporal_load(&vin[i]);
+        }
+    }
+    bool bad = false;
+
+    for (uint32_t tile = 0; tile < len; tile += T) {
+        const int tn = (int)min((uint32_t)T, len - tile);
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) L.cnt[b] = 0u;
+        __syncthreads();
+        uint32_t bk[ITEMS], rk[ITEMS];
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const int s = it * BLOCK + threadIdx.x;
+            if (s < tn) {
+                bk[it] = (radix_hash(k[it]) >> shift) & bmask;
+                rk[it] = atomicAdd(&L.cnt[bk[it]], 1u);
+            }
+        }
+        __syncthreads();
+        lds_exclusive_scan<BLOCK>(L.cnt, L.off, nbk, L.wtmp);
+        // tuples of bucket b written this tile: up to its last whole chunk
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
+            const uint32_t a0 = (uint32_t)L.cur[b] & (cs - 1u);
+            const uint32_t e = a0 + L.ccnt[b] + L.cnt[b];
+            L.nw[b] = e >= cs ? (e & ~(cs - 1u)) - a0 : 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const int s = it * BLOCK + threadIdx.x;
+            if (s < tn) {
+                const uint32_t p = L.off[bk[it]] + rk[it];
+                L.skey[p] = k[it];
+                L.sval[p] = v[it];
+            }
+        }
+        // prefetch the next tile while this one is written out
+        const uint32_t next = tile + T;
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const uint32_t i = next + (uint32_t)it * BLOCK + threadIdx.x;
+            if (i < len) {
+                k[it] = __builtin_nontemporal_load(&kin[i]);
+                v[it] = __builtin_nontemporal_load(&vin[i]);
+            }
+        }
+        __syncthreads();
+        // phase A: carried tuples inside the written range go out
+        for (int idx = threadIdx.x; idx < (nbk << cs_log2); idx += BLOCK) {
+            const int b = idx >> cs_log2;
+            const uint32_t i = (uint32_t)idx & (cs - 1u);
+            if (i < L.ccnt[b] && i < L.nw[b]) {
+                const uint64_t g = L.cur[b] + i;
+                if (g < out_cap) {
+                    okey[g] = L.ck[idx];
+                    oval[g] = L.cv[idx];
+                } else {
+                    bad = true;
+                }
+            }
+        }
+        __syncthreads();
+        // phase B: the tile's tuples go out, or into the carry
+        for (int s = threadIdx.x; s < tn; s += BLOCK) {
+            const uint32_t kk = L.skey[s];
+            const uint32_t b = (radix_hash(kk) >> shift) & bmask;
+            const uint32_t rel = L.ccnt[b] + (uint32_t)s - L.off[b];
+            const uint32_t w_b = L.nw[b];
+            if (rel < w_b) {
+                const uint64_t g = L.cur[b] + rel;
+                if (g < out_cap) {
+                    okey[g] = kk;
+                    oval[g] = L.sval[s];
+                } else {
+                    bad = true;
+                }
+            } else {
+                const uint32_t slot = (b << cs_log2) + rel - w_b;
+                L.ck[slot] = kk;
+                L.cv[slot] = L.sval[s];
+            }
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
+            L.cur[b] += L.nw[b];
+            L.ccnt[b] = L.ccnt[b] + L.cnt[b] - L.nw[b];
+        }
+        __syncthreads();
+    }
+    // the remaining carries (partial chunks at the range ends)
+    for (int idx = threadIdx.x; idx < (nbk << cs_log2); idx += BLOCK) {
+        const int b = idx >> cs_log2;
+        const uint32_t i = (uint32_t)idx & (cs - 1u);
+        if (i < L.ccnt[b]) {
+            const uint64_t g = L.cur[b] + i;
+            if (g < out_cap) {
+                okey[g] = L.ck[idx];
+                oval[g] = L.cv[idx];
+            } else {
+                bad = true;
+            }
+        }
+    }
+    return bad;
+}
+
+// ---------------------------------------------------------------- Bloom build
+// seg[] (LDS, wps 64-bit blocks, zeroed by the caller) |= the block masks of
+// rkey[lo, hi); all keys belong to the segment starting at block segbase.
+template <int BLOCK>
+__device__ __forceinline__ void bloom_build_range(const uint32_t* __restrict__ rkey, uint64_t lo,
+                                                  uint64_t hi, const BloomGeom& g, uint64_t segbase,
+                                                  unsigned long long* seg)
+{
+    constexpr int U = 8;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += (uint64_t)U * BLOCK) {
+        uint32_t kk[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t j = i + (uint64_t)u * BLOCK;
+            kk[u] = (j < hi) ? __builtin_nontemporal_load(&rkey[j]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (i + (uint64_t)u * BLOCK < hi) {
+                const uint32_t w = (uint32_t)(bloom_block(kk[u], g) - segbase);
+                atomicOr(&seg[w], (unsigned long long)bloom_mask(kk[u], g.k));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- Bloom filter S -> S'
+// One tile [tile, tile + BLOCK*ITEMS) of skey/sval (clipped to n): every key
+// tests its 64-bit block; survivors are compacted with a wave ballot + mbcnt
+// per (item, wave) group, an LDS scan over the groups and one global atomic on
+// *count per tile, and written to okey/oval at that position.  Values are read
+// only for survivors.  gcnt holds ITEMS * BLOCK/64 entries, s_base one.
+// Returns true if a write would pass out_cap.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey,
+                                               const uint32_t* __restrict__ sval, uint64_t tile, uint64_t n,
+                                               const uint64_t* __restrict__ filter, const BloomGeom& g,
+                                               uint32_t* __restrict__ okey, uint32_t* __restrict__ oval,
+                                               uint64_t out_cap, uint64_t* __restrict__ count, uint32_t* gcnt,
+                                               uint64_t* s_base)
+{
+    constexpr int NW = BLOCK / WAVE;
+    constexpr int NG = ITEMS * NW;  // groups per tile
+    static_assert(NG <= 2 * WAVE, "group scan handles at most 128 groups");
+    const int wid = threadIdx.x / WAVE;
+    const uint32_t lane = lane_id();
+    uint32_t k[ITEMS];
+    bool pass[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
+        k[it] = (i < n) ? __builtin_nontemporal_load(&skey[i]) : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
+        bool ok = false;
+        if (i < n) {
+            const uint64_t m = bloom_mask(k[it], g.k);
+            ok = (filter[bloom_block(k[it], g)] & m) == m;
+        }
+        pass[it] = ok;
+    }
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint64_t m = __ballot(pass[it]);
+        rank[it] = lanes_below(m);
+        if (lane == 0) gcnt[it * NW + wid] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (wid == 0) {
+        // exclusive scan of NG (<= 128) group counts: two entries per lane
+        const uint32_t a0 = (2 * lane < (uint32_t)NG) ? gcnt[2 * lane] : 0u;
+        const uint32_t a1 = (2 * lane + 1 < (uint32_t)NG) ? gcnt[2 * lane + 1] : 0u;
+        const uint32_t inc = wave_inclusive_scan<uint32_t>(a0 + a1);
+        const uint32_t ex = inc - a0 - a1;
+        if (2 * lane < (uint32_t)NG) gcnt[2 * lane] = ex;
+        if (2 * lane + 1 < (uint32_t)NG) gcnt[2 * lane + 1] = ex + a0;
+        const uint32_t total = __shfl(inc, WAVE - 1, WAVE);
+        if (lane == 0)
+            *s_base = total ? atomicAdd((unsigned long long*)count, (unsigned long long)total) : 0ull;
+    }
+    __syncthreads();
+    bool bad = false;
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        if (pass[it]) {
+            const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
+            const uint64_t pos = *s_base + gcnt[it * NW + wid] + rank[it];
+            if (pos < out_cap) {
+                okey[pos] = k[it];
+                oval[pos] = sval[i];
+            } else {
+                bad = true;
+            }
+        }
+    }
+    __syncthreads();  // gcnt / s_base are rewritten by the next tile
+    return bad;
+}
+
+// ---------------------------------------------------------------- build + probe of one work item
+// The build side's chunk (bn <= CAP tuples) is counting-sorted into an LDS
+// table by the hash bits below the radix bits: binfo[t] packs (start, count)
+// of bucket t, hkey[] holds the keys bucket by bucket and hidx[] their row in
+// the chunk (6 bytes per build tuple).  A probe key reads one binfo word, then
+// scans a short contiguous run of keys: no pointer chasing.  The probe side
+// (pn keys) streams through in rounds of BLOCK*PITEMS keys (the next round is
+// loaded while the current one is processed).  Each round counts its matches,
+// reserves its rows with one global atomic after a block scan, then writes
+// them.  Both values of a row are read only for matches: the build side's from
+// its (cache-resident) chunk, the probe side's from its slice.  Rows past
+// out_cap are counted, not written.
+template <int BLOCK, int CAP>
+struct ProbeLds {
+    uint32_t hkey[CAP];
+    uint16_t hidx[CAP];
+    uint32_t binfo[CAP / 2];
+    uint32_t wtmp[BLOCK / WAVE + 1];
+    unsigned long long s_base;
+};
+
+template <int BLOCK, int CAP, int PITEMS>
+__device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32_t* __restrict__ bkp,
+                                           const uint32_t* __restrict__ bvp, int bn,
+                                           const uint32_t* __restrict__ pkp, const uint32_t* __restrict__ pvp,
+                                           uint32_t pn, bool build_r, int bits,
+                                           uint64_t* __restrict__ out_count, uint64_t out_cap,
+                                           uint32_t* __restrict__ o_key, uint32_t* __restrict__ o_rval,
+                                           uint32_t* __restrict__ o_sval)
+{
+    constexpr int NBMAX = CAP / 2;
+    constexpr int ROUND = BLOCK * PITEMS;
+    constexpr int BPT = CAP / BLOCK;  // build tuples per thread
+    static_assert(CAP % BLOCK == 0 && CAP <= 65535, "table indices are 16-bit");
+
+    // table geometry: 2^tb buckets, about 1..2 build tuples per bucket
+    int tb = 0;
+    while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= NBMAX) tb++;
+    const int nbk = 1 << tb;
+    const int tshift = 32 - tb;
+    for (int i = threadIdx.x; i < nbk; i += BLOCK) L.binfo[i] = 0u;
+    __syncthreads();
+    // build: rank each tuple in its bucket, scan the counts, place;
+    // bucket (< 2^12) and rank (< 2^16) of each build tuple, packed
+    uint32_t bkey[BPT], btr[BPT];
+#pragma unroll
+    for (int u = 0; u < BPT; u++) {
+        const int i = u * BLOCK + threadIdx.x;
+        bkey[u] = (i < bn) ? bkp[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < BPT; u++) {
+        const int i = u * BLOCK + threadIdx.x;
+        if (i < bn) {
+            const uint32_t t = tb ? ((radix_hash(bkey[u]) << bits) >> tshift) : 0u;
+            btr[u] = (atomicAdd(&L.binfo[t], 1u) << 16) | t;
+        }
+    }
+    __syncthreads();
+    {
+        // exclusive scan of the counts; binfo[t] = start | count << 16
+        const int per_t = (nbk + BLOCK - 1) / BLOCK;
+        const int lo_t = min((int)threadIdx.x * per_t, nbk), hi_t = min(lo_t + per_t, nbk);
+        uint32_t sum = 0;
+        for (int t = lo_t; t < hi_t; t++) sum += L.binfo[t];
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan<BLOCK, uint32_t>(sum, L.wtmp, &tot);
+        for (int t = lo_t; t < hi_t; t++) {
+            const uint32_t c = L.binfo[t];
+            L.binfo[t] = ex | (c << 16);
+            ex += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < BPT; u++) {
+        const int i = u * BLOCK + threadIdx.x;
+        if (i < bn) {
+            const uint32_t q = (L.binfo[btr[u] & 0xFFFFu] & 0xFFFFu) + (btr[u] >> 16);
+            L.hkey[q] = bkey[u];
+            L.hidx[q] = (uint16_t)i;
+        }
+    }
+    __syncthreads();
+
+    uint32_t nk[PITEMS];
+#pragma unroll
+    for (int it = 0; it < PITEMS; it++) {
+        const uint32_t idx = (uint32_t)it * BLOCK + threadIdx.x;
+        nk[it] = (idx < pn) ? pkp[idx] : 0u;
+    }
+    for (uint32_t base = 0; base < pn; base += ROUND) {
+        uint32_t kk[PITEMS], info[PITEMS];
+#pragma unroll
+        for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
+        // prefetch the next round
+#pragma unroll
+        for (int it = 0; it < PITEMS; it++) {
+            const uint32_t idx = base + ROUND + (uint32_t)it * BLOCK + threadIdx.x;
+            if (idx < pn) nk[it] = pkp[idx];
+        }
+#pragma unroll
+        for (int it = 0; it < PITEMS; it++) {
+            const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
+            info[it] = (idx < pn) ? L.binfo[tb ? ((radix_hash(kk[it]) << bits) >> tshift) : 0u] : 0u;
+        }
+        // count
+        uint32_t c = 0, hitmask = 0;
+#pragma unroll
+        for (int it = 0; it < PITEMS; it++) {
+            const uint32_t st = info[it] & 0xFFFFu, en = st + (info[it] >> 16);
+            uint32_t m = 0;
+            for (uint32_t q = st; q < en; q++) m += (L.hkey[q] == kk[it]);
+            c += m;
+            hitmask |= (m ? 1u : 0u) << it;
+        }
+        uint32_t total;
+        const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, L.wtmp, &total);
+        if (total == 0) continue;  // uniform
+        if (threadIdx.x == 0)
+            L.s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
+        __syncthreads();
+        uint64_t w = L.s_base + pre;
+#pragma unroll
+        for (int it = 0; it < PITEMS; it++) {
+            if (hitmask & (1u << it)) {
+                const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
+                const uint32_t pval = pvp[idx];
+                const uint32_t st = info[it] & 0xFFFFu, en = st + (info[it] >> 16);
+                for (uint32_t q = st; q < en; q++) {
+                    if (L.hkey[q] == kk[it]) {
+                        if (w < out_cap) {
+                            const uint32_t bval = bvp[L.hidx[q]];
+                            o_key[w] = kk[it];
+                            o_rval[w] = build_r ? bval : pval;
+                            o_sval[w] = build_r ? pval : bval;
+                        }
+                        w++;
+                    }
+                }
+            }
+        }
+    }
+    // the caller synchronises before the next item rebuilds the table
+}
+
+// Work-item split of one sub-bucket (vlsplit).  A sub-bucket whose smaller side
+// fits one LDS tile and whose larger side fits one slice is "regular": one work
+// item, indexed by the sub-bucket itself.  Larger ones are split: the build
+// side into ceil(nb / tile) chunks, the probe side into
+// min(ceil(np / slice), max_slices) slices, one explicit item per (chunk, slice).
+__device__ __forceinline__ uint32_t plan_items(uint64_t nr, uint64_t ns, uint32_t tile, uint32_t slice,
+                                               uint32_t max_slices, uint32_t& chunks, uint32_t& slices)
+{
+    chunks = slices = 0;
+    if (nr == 0 || ns == 0) return 0;
+    const uint64_t nb = min(nr, ns), np = max(nr, ns);
+    if (nb <= tile && np <= slice) return 0;
+    chunks = (uint32_t)((nb + tile - 1) / tile);
+    slices = (uint32_t)min<uint64_t>((np + slice - 1) / slice, max_slices);
+    return chunks * slices;
+}
+
+}  // namespace bj

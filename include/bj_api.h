@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define BJ_API_VERSION 1
+#define BJ_API_VERSION 2
 
 /* ---- error codes ------------------------------------------------------- */
 #define BJ_OK              0
@@ -267,6 +267,148 @@ void bj_main_args_default(bj_main_args *a);
 /* Returns BJ_OK and fills *res (rows stay on the device of the internal
  * context, which is destroyed on return: only the counts remain valid). */
 int  bj_gpu_main(const bj_main_args *a, bj_result *res);
+
+/* ---- batched form ----------------------------------------------------------
+ * Many independent joins of uneven sizes run as one sequence of launches: one
+ * launch per phase covers every join (each join partitioned in one level of
+ * its own radix bits into its region of shared arenas), and the build + probe
+ * is one persistent launch per LDS tile size over the sub-buckets of all
+ * joins.  The sequence is captured once in a hipGraph and replayed by
+ * bj_batch_run.  Radix bits, LDS tile and Bloom bits per key of each join come
+ * from its spec, else from the context's autotuner table, else from a size
+ * rule.  Each join's result equals bj_join's on the same inputs (same count,
+ * same row multiset).  No counterpart in the reference, which runs one join
+ * per gpu_main call (src/hash_kernel.cu:103-1654). */
+typedef struct bj_join_spec {
+    const uint32_t *r_key, *r_val;   /* device pointers                          */
+    const uint32_t *s_key, *s_val;
+    int64_t r_size, s_size;          /* each < 2^31                              */
+    int32_t radix_bits;              /* 1..11; -1 = autotuner / size rule        */
+    int32_t tile;                    /* 4096 or 8192; 0 = autotuner / 8192       */
+    double  bf_bits_per_key;         /* > 0: filter on; 0: off; < 0: autotuner   */
+    int32_t num_hashes;              /* 1..8; -1 = round(bits_per_key * ln 2)    */
+    double  answer_scale;            /* row capacity = max(R,S) * max(1, scale)  */
+    double  match_hint;              /* expected fraction of S with a match, used
+                                        only to pick the autotuner's table entry;
+                                        < 0 = unknown (taken as 0.1)             */
+} bj_join_spec;
+
+typedef struct bj_join_out {
+    uint64_t match_count;            /* exact                                    */
+    uint64_t rows_written;           /* min(match_count, capacity)               */
+    uint64_t capacity;
+    uint64_t s_survivors;            /* |S'| after the filter (= |S| without)    */
+    int32_t  truncated;
+    int32_t  radix_bits, tile, bf_k; /* the configuration used                   */
+    double   bf_bits_per_key;        /* 0 = filter off                           */
+    /* rows (SoA, device, unordered), owned by the batch until bj_batch_destroy */
+    uint32_t *key, *r_val, *s_val;
+} bj_join_out;
+
+/* One autotuner table entry.  kind 0: radix bits and tile for joins whose
+ * smaller side is about 2^lg tuples; kind 1: filter bits per key for joins with
+ * |S|/|R| about 2^sr and selectivity class sel (0: <= 3%, 1: <= 15%, 2: <= 35%,
+ * 3: more). */
+typedef struct bj_tune_entry {
+    int32_t kind;
+    int32_t lg, sr, sel;
+    int32_t radix_bits, tile;
+    double  bf_bits_per_key;
+    double  ms;                      /* best measured time of the tuning batch   */
+} bj_tune_entry;
+
+typedef struct bj_batch bj_batch;
+
+void bj_join_spec_default(bj_join_spec *s);
+/* Configuration (radix_bits, tile, bf_bits_per_key, bf_k, capacity fields of
+ * cfg[i]) and device footprint a batch would use; ctx may be NULL (size rules
+ * only, no device needed). */
+int  bj_batch_plan(const bj_ctx *ctx, int n, const bj_join_spec *specs, bj_join_out *cfg,
+                   uint64_t *workspace_bytes);
+/* Plans the batch, allocates its workspace and (use_graph) captures its launch
+ * sequence in a hipGraph.  The input pointers of specs must stay valid while
+ * the batch is used; their contents may change between runs. */
+int  bj_batch_create(bj_ctx *ctx, int n, const bj_join_spec *specs, int use_graph, bj_batch **out);
+/* Runs the batch once; fills outs[0..n) in the order of the specs.  device_ms
+ * (may be NULL) receives the device time of the launch sequence. */
+int  bj_batch_run(bj_batch *b, bj_join_out *outs, double *device_ms);
+int  bj_batch_destroy(bj_batch *b);
+
+/* Autotuner: times candidate configurations on synthetic batches of the size
+ * and selectivity classes that specs fall into (classes already in the table
+ * are kept) and records the fastest in the context's table. */
+int  bj_autotune(bj_ctx *ctx, int n, const bj_join_spec *specs, int *entries_tuned);
+int  bj_tune_get(const bj_ctx *ctx, bj_tune_entry *out, int cap, int *n);
+int  bj_tune_set(bj_ctx *ctx, const bj_tune_entry *in, int n);
+
+/* ---- serving form ----------------------------------------------------------
+ * A request loop over device-resident relations.  bj_server_add_relation
+ * partitions R once (and builds its filter) into server-owned memory; each
+ * request joins a caller's S against one resident R, exactly as bj_join would
+ * join them.  Requests are admitted in arrival order as soon as a slot (a
+ * stream of the server) is free and the request's footprint (its S workspace
+ * and its rows) fits the server's HBM arena: requests that arrive while others
+ * run start as slots and memory free up, and their kernels overlap on the
+ * device (continuous batching).  A request's rows stay in the arena until
+ * bj_server_release.  The server's scheduler runs on its own host thread; all
+ * entry points may be called from any one other thread.  No counterpart in the
+ * reference. */
+typedef struct bj_server_opts {
+    int32_t  slots;            /* requests on the device at once, 1..8 (default 3)  */
+    uint64_t arena_bytes;      /* device memory for request workspaces and rows;
+                                  0 = half of the free memory at creation          */
+    int32_t  use_bf;           /* a filter per relation (default 1)                 */
+    double   bf_bits_per_key;  /* default 10                                        */
+    int32_t  num_hashes;       /* default 3                                         */
+    int32_t  radix_bits;       /* per relation, 1..15; -1 = from |R| (default)      */
+    double   answer_scale;     /* row capacity of a request = |S| * max(1, scale)   */
+} bj_server_opts;
+
+typedef struct bj_request_result {
+    uint64_t id;
+    int32_t  status;           /* BJ_OK or a BJ_ERR_* code                          */
+    int32_t  relation;
+    uint64_t s_size;
+    uint64_t match_count, rows_written, capacity, s_survivors;
+    int32_t  truncated;
+    double   t_submit, t_start, t_end;  /* host seconds since bj_server_create;
+                                           latency = t_end - t_submit            */
+    double   device_ms;        /* first kernel to last, on the device               */
+    uint32_t *key, *r_val, *s_val;  /* rows (device, unordered), until release     */
+} bj_request_result;
+
+typedef struct bj_server_stats {
+    uint64_t submitted, completed, released;
+    int32_t  max_in_flight;    /* most requests on the device at one time           */
+    uint64_t arena_bytes, arena_peak_bytes, relation_bytes;
+} bj_server_stats;
+
+typedef struct bj_server bj_server;
+
+void bj_server_opts_default(bj_server_opts *o);
+int  bj_server_create(int device, const bj_server_opts *o, bj_server **out);
+/* Partitions R (device pointers; not retained) into the server: *rel_id. */
+int  bj_server_add_relation(bj_server *s, const uint32_t *r_key, const uint32_t *r_val,
+                            int64_t n_r, int32_t *rel_id);
+/* Arena bytes a request of n_s tuples against relation rel occupies. */
+int  bj_server_request_bytes(const bj_server *s, int32_t rel, int64_t n_s, uint64_t *bytes);
+/* Queues a request (device pointers, read while it runs); never blocks.
+ * BJ_ERR_SIZE if its footprint exceeds the whole arena. */
+int  bj_server_submit(bj_server *s, int32_t rel, const uint32_t *s_key, const uint32_t *s_val,
+                      int64_t n_s, uint64_t *id);
+/* Completed requests not yet returned by poll (at most cap); never blocks. */
+int  bj_server_poll(bj_server *s, bj_request_result *out, int cap, int *n);
+/* Blocks until request id has completed. */
+int  bj_server_wait(bj_server *s, uint64_t id, bj_request_result *out);
+/* Copies the first n rows of a completed request to host arrays (n <= rows_written). */
+int  bj_server_copy_rows(bj_server *s, const bj_request_result *r, uint32_t *key, uint32_t *r_val,
+                         uint32_t *s_val, uint64_t n);
+/* Frees a completed request's rows. */
+int  bj_server_release(bj_server *s, uint64_t id);
+/* Blocks until every submitted request has completed. */
+int  bj_server_drain(bj_server *s);
+int  bj_server_stats_get(bj_server *s, bj_server_stats *out);
+int  bj_server_destroy(bj_server *s);
 
 #ifdef __cplusplus
 }
