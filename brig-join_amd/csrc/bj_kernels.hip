@@ -418,10 +418,11 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
     uint64_t* dst0 = lds64;                       // [P2] sub-bucket start + reserved base
     uint32_t* skey = (uint32_t*)(dst0 + P2);      // [T]
     uint32_t* sval = skey + T;                    // [T]
-    uint32_t* cnt = sval + T;                     // [P2]
-    uint32_t* off = cnt + P2;                     // [P2]
+    uint32_t* cnt0 = sval + T;                    // [2 * P2] survivors per sub-bucket (two tiles)
+    uint32_t* off = cnt0 + 2 * P2;                // [P2]
     uint32_t* wtmp = off + P2;                    // [BLOCK/64 + 1]
     __shared__ uint32_t s_total;
+    const bool small = P2 <= 8 * WAVE;
 
     const int xcd = blockIdx.x % NXCD;
     const int W = gridDim.x / NXCD;
@@ -429,17 +430,23 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
     if (j >= W) return;  // grid is a multiple of NXCD; uniform per workgroup
     bool bad = false;
     uint32_t surv_total = 0;
+    for (int b = threadIdx.x; b < 2 * P2; b += BLOCK) cnt0[b] = 0u;
+    __syncthreads();
+    uint32_t par = 0;
 
+    // Per tile three barriers: test + rank | offsets and reservations (one wave;
+    // it also clears the other tile's counts) | staging | out (runs into the
+    // next tile's test).
     for (int p = xcd; p < P1; p += NXCD) {
         const uint64_t lo = sb_start[(uint64_t)p << bits2];
         const uint64_t hi = sb_start[(uint64_t)(p + 1) << bits2];
         const uint64_t chunk = (hi - lo + W - 1) / W;
         const uint64_t clo = min(lo + (uint64_t)j * chunk, hi);
         const uint64_t chi = min(clo + chunk, hi);
-        for (uint64_t tile = clo; tile < chi; tile += T) {
+        for (uint64_t tile = clo; tile < chi; tile += T, par ^= 1u) {
             const int tn = (int)min((uint64_t)T, chi - tile);
-            for (int b = threadIdx.x; b < P2; b += BLOCK) cnt[b] = 0u;
-            __syncthreads();
+            uint32_t* cnt = cnt0 + (par ? P2 : 0);
+            uint32_t* cnext = cnt0 + (par ? 0 : P2);
             uint32_t k[ITEMS], bk[ITEMS], rk[ITEMS];
             bool pass[ITEMS];
 #pragma unroll
@@ -448,14 +455,18 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
                 // streamed once: non-temporal, so the filter slice keeps the L2
                 k[it] = (s < tn) ? __builtin_nontemporal_load(&ikey[tile + s]) : 0u;
             }
+            // all filter words of the tile in flight before any is tested
+            uint64_t fw[ITEMS];
 #pragma unroll
             for (int it = 0; it < ITEMS; it++) {
                 const int s = it * BLOCK + threadIdx.x;
-                bool ok = false;
-                if (s < tn) {
-                    const uint64_t m = bloom_mask(k[it], g.k);
-                    ok = (filter[bloom_block(k[it], g)] & m) == m;
-                }
+                fw[it] = (s < tn) ? filter[bloom_block(k[it], g)] : 0ull;
+            }
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++) {
+                const int s = it * BLOCK + threadIdx.x;
+                const uint64_t m = bloom_mask(k[it], g.k);
+                const bool ok = s < tn && (fw[it] & m) == m;
                 pass[it] = ok;
                 if (ok) {
                     bk[it] = (radix_hash(k[it]) >> shift) & bmask;
@@ -463,21 +474,46 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
                 }
             }
             __syncthreads();
-            const uint32_t tot = lds_exclusive_scan<BLOCK>(cnt, off, P2, wtmp);
-            if (threadIdx.x == 0) s_total = tot;
-            for (int b = threadIdx.x; b < P2; b += BLOCK) {
-                const uint64_t sb = ((uint64_t)p << bits2) + b;
-                const uint32_t c = cnt[b];
-                dst0[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
+            if (small) {
+                if (threadIdx.x < WAVE) {
+                    const int per = (P2 + WAVE - 1) / WAVE;
+                    const int b0 = min((int)threadIdx.x * per, P2), b1 = min(b0 + per, P2);
+                    uint32_t sum = 0;
+                    for (int b = b0; b < b1; b++) sum += cnt[b];
+                    const uint32_t inc = wave_inclusive_scan<uint32_t>(sum);
+                    uint32_t ex = inc - sum;
+                    for (int b = b0; b < b1; b++) {
+                        const uint32_t c = cnt[b];
+                        off[b] = ex;
+                        ex += c;
+                        const uint64_t sb = ((uint64_t)p << bits2) + b;
+                        dst0[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
+                        cnext[b] = 0u;
+                    }
+                    if (threadIdx.x == WAVE - 1) s_total = inc;
+                }
+            } else {
+                const uint32_t tot = lds_exclusive_scan<BLOCK>(cnt, off, P2, wtmp);
+                if (threadIdx.x == 0) s_total = tot;
+                for (int b = threadIdx.x; b < P2; b += BLOCK) {
+                    const uint64_t sb = ((uint64_t)p << bits2) + b;
+                    const uint32_t c = cnt[b];
+                    dst0[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
+                    cnext[b] = 0u;
+                }
             }
+            // survivors' values: loads first, then the LDS stores
+            uint32_t vv[ITEMS];
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++)
+                if (pass[it]) vv[it] = __builtin_nontemporal_load(&ival[tile + it * BLOCK + threadIdx.x]);
             __syncthreads();
 #pragma unroll
             for (int it = 0; it < ITEMS; it++) {
                 if (pass[it]) {
-                    const int s = it * BLOCK + threadIdx.x;
                     const uint32_t q = off[bk[it]] + rk[it];
                     skey[q] = k[it];
-                    sval[q] = __builtin_nontemporal_load(&ival[tile + s]);
+                    sval[q] = vv[it];
                 }
             }
             __syncthreads();
@@ -494,7 +530,6 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
                 }
             }
             surv_total += ts;
-            __syncthreads();
         }
     }
     if (threadIdx.x == 0 && surv_total)
@@ -750,7 +785,7 @@ static int log2i(int x)
 size_t scatter_lds_bytes(int nbk)
 {
     const int cs = scatter_carry(nbk);
-    return (size_t)nbk * 8 + (size_t)nbk * cs * 8 + (size_t)SC_BLOCK * SC_ITEMS * 8 + (size_t)nbk * 16 +
+    return (size_t)nbk * 8 + (size_t)nbk * cs * 8 + (size_t)SC_BLOCK * SC_ITEMS * 8 + (size_t)nbk * 20 +
            (SC_BLOCK / WAVE + 1) * 4;
 }
 
@@ -801,7 +836,7 @@ hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const ui
                             uint64_t* err, int num_cu, hipStream_t st)
 {
     const int P2 = 1 << bits2;
-    const size_t lds = (size_t)P2 * 8 + (size_t)FL_BLOCK * FL_ITEMS * 8 + (size_t)P2 * 8 +
+    const size_t lds = (size_t)P2 * 8 + (size_t)FL_BLOCK * FL_ITEMS * 8 + (size_t)P2 * 12 +
                        (FL_BLOCK / WAVE + 1) * 4;
     const int W = std::max(1, num_cu / 8) * 2;  // workgroups per XCD
     k_filter_l1<FL_BLOCK, FL_ITEMS><<<8 * W, FL_BLOCK, lds, st>>>(ikey, ival, sb_start, bits, bits2, filter,

@@ -267,7 +267,7 @@ void scheduler(bj_server* s)
             changed = true;
         }
         // admit queued requests in arrival order while a slot and arena room are free
-        while (!s->queue.empty()) {
+        while (!s->stop && !s->queue.empty()) {
             Slot* free_slot = nullptr;
             for (Slot& sl : s->slots)
                 if (!sl.cur) {
@@ -301,7 +301,7 @@ void scheduler(bj_server* s)
             changed = true;
         }
         if (changed) s->cv_done.notify_all();
-        if (s->stop && s->queue.empty() && s->in_flight == 0) break;
+        if (s->stop && s->in_flight == 0) break;  // queued requests are dropped
         if (s->in_flight == 0 && s->queue.empty())
             s->cv_work.wait(lk);
         else

@@ -46,7 +46,8 @@ struct ScatterLds {
     uint32_t* cv;    // [nbk * cs] carried values
     uint32_t* skey;  // [T]
     uint32_t* sval;  // [T]
-    uint32_t* cnt;   // [nbk] tile tuples
+    uint32_t* cnt0;  // [nbk] tile tuples (even tiles)
+    uint32_t* cnt1;  // [nbk] tile tuples (odd tiles)
     uint32_t* off;   // [nbk] staging offset
     uint32_t* ccnt;  // [nbk] carried tuples
     uint32_t* nw;    // [nbk] tuples written this tile
@@ -63,18 +64,43 @@ __device__ __forceinline__ ScatterLds scatter_lds(uint64_t* lds64, int nbk, int 
     L.cv = L.ck + (nbk << cs_log2);
     L.skey = L.cv + (nbk << cs_log2);
     L.sval = L.skey + T;
-    L.cnt = L.sval + T;
-    L.off = L.cnt + nbk;
+    L.cnt0 = L.sval + T;
+    L.cnt1 = L.cnt0 + nbk;
+    L.off = L.cnt1 + nbk;
     L.ccnt = L.off + nbk;
     L.nw = L.ccnt + nbk;
     L.wtmp = L.nw + nbk;
     return L;
 }
 
+// Bucket b after a tile: the nw[b] tuples written leave the carry / tile, the rest
+// (ccnt + cnt - nw) stay carried.
+__device__ __forceinline__ void scatter_retire(const ScatterLds& L, uint32_t* cprev, int b)
+{
+    const uint32_t w = L.nw[b];
+    L.cur[b] += w;
+    L.ccnt[b] = L.ccnt[b] + cprev[b] - w;
+    cprev[b] = 0u;
+}
+
+// Tuples of bucket b written this tile: up to its last whole chunk.
+__device__ __forceinline__ uint32_t scatter_nw(const ScatterLds& L, int b, uint32_t c, uint32_t cs)
+{
+    const uint32_t a0 = (uint32_t)L.cur[b] & (cs - 1u);
+    const uint32_t e = a0 + L.ccnt[b] + c;
+    return e >= cs ? (e & ~(cs - 1u)) - a0 : 0u;
+}
+
 // Scatters kin/vin[0, len) by bucket (radix_hash(key) >> shift) & bmask into
 // okey/oval at the cursors L.cur[] (initialised by the caller; this function
 // begins with a barrier).  Writes are bounds-checked against out_cap; returns
 // true on a violation.
+//
+// Per tile three barriers: ranks (LDS atomics) | offsets (one wave scans the
+// counts, retires the previous tile's buckets and sizes this tile's writes) |
+// staging + carried tuples out + next tile's loads | the tile's tuples out or
+// into the carry (which runs into the next tile's ranks).  Counts alternate
+// between two arrays so the next tile's ranks need no clearing barrier.
 template <int BLOCK, int ITEMS>
 __device__ __forceinline__ bool scatter_range(const ScatterLds& L, const uint32_t* __restrict__ kin,
                                               const uint32_t* __restrict__ vin, uint32_t len, int nbk,
@@ -83,8 +109,15 @@ __device__ __forceinline__ bool scatter_range(const ScatterLds& L, const uint32_
                                               uint64_t out_cap)
 {
     constexpr int T = BLOCK * ITEMS;
+    constexpr int SMALL = 8;  // buckets per lane of the one-wave scan
     const uint32_t cs = 1u << cs_log2;
-    for (int b = threadIdx.x; b < nbk; b += BLOCK) L.ccnt[b] = 0u;
+    const bool small = nbk <= SMALL * WAVE;
+    for (int b = threadIdx.x; b < nbk; b += BLOCK) {
+        L.ccnt[b] = 0u;
+        L.cnt0[b] = 0u;
+        L.cnt1[b] = 0u;
+        L.nw[b] = 0u;
+    }
     // cur/ccnt are read by other threads; an empty range skips the tile loop's barriers
     __syncthreads();
 
@@ -98,27 +131,46 @@ __device__ __forceinline__ bool scatter_range(const ScatterLds& L, const uint32_
         }
     }
     bool bad = false;
+    uint32_t par = 0;
 
-    for (uint32_t tile = 0; tile < len; tile += T) {
+    for (uint32_t tile = 0; tile < len; tile += T, par ^= 1u) {
+        // cnt1 follows cnt0: select by offset (a select between the two struct
+        // fields would put the struct in scratch memory)
+        uint32_t* cnt = L.cnt0 + (par ? nbk : 0);
+        uint32_t* cprev = L.cnt0 + (par ? 0 : nbk);
         const int tn = (int)min((uint32_t)T, len - tile);
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) L.cnt[b] = 0u;
-        __syncthreads();
         uint32_t bk[ITEMS], rk[ITEMS];
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
             const int s = it * BLOCK + threadIdx.x;
             if (s < tn) {
                 bk[it] = (radix_hash(k[it]) >> shift) & bmask;
-                rk[it] = atomicAdd(&L.cnt[bk[it]], 1u);
+                rk[it] = atomicAdd(&cnt[bk[it]], 1u);
             }
         }
         __syncthreads();
-        lds_exclusive_scan<BLOCK>(L.cnt, L.off, nbk, L.wtmp);
-        // tuples of bucket b written this tile: up to its last whole chunk
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
-            const uint32_t a0 = (uint32_t)L.cur[b] & (cs - 1u);
-            const uint32_t e = a0 + L.ccnt[b] + L.cnt[b];
-            L.nw[b] = e >= cs ? (e & ~(cs - 1u)) - a0 : 0u;
+        if (small) {
+            if (threadIdx.x < WAVE) {
+                // lane l owns buckets [l*per, l*per + per)
+                const int per = (nbk + WAVE - 1) / WAVE;
+                const int b0 = min((int)threadIdx.x * per, nbk), b1 = min(b0 + per, nbk);
+                uint32_t sum = 0;
+                for (int b = b0; b < b1; b++) {
+                    scatter_retire(L, cprev, b);
+                    sum += cnt[b];
+                }
+                uint32_t ex = wave_inclusive_scan<uint32_t>(sum) - sum;
+                for (int b = b0; b < b1; b++) {
+                    const uint32_t c = cnt[b];
+                    L.off[b] = ex;
+                    ex += c;
+                    L.nw[b] = scatter_nw(L, b, c, cs);
+                }
+            }
+        } else {
+            for (int b = threadIdx.x; b < nbk; b += BLOCK) scatter_retire(L, cprev, b);
+            lds_exclusive_scan<BLOCK>(cnt, L.off, nbk, L.wtmp);
+            for (int b = threadIdx.x; b < nbk; b += BLOCK) L.nw[b] = scatter_nw(L, b, cnt[b], cs);
         }
         __syncthreads();
 #pragma unroll
@@ -140,8 +192,7 @@ __device__ __forceinline__ bool scatter_range(const ScatterLds& L, const uint32_
                 v[it] = __builtin_nontemporal_load(&vin[i]);
             }
         }
-        __syncthreads();
-        // phase A: carried tuples inside the written range go out
+        // carried tuples inside the written range go out
         for (int idx = threadIdx.x; idx < (nbk << cs_log2); idx += BLOCK) {
             const int b = idx >> cs_log2;
             const uint32_t i = (uint32_t)idx & (cs - 1u);
@@ -156,7 +207,7 @@ __device__ __forceinline__ bool scatter_range(const ScatterLds& L, const uint32_
             }
         }
         __syncthreads();
-        // phase B: the tile's tuples go out, or into the carry
+        // the tile's tuples go out, or into the carry
         for (int s = threadIdx.x; s < tn; s += BLOCK) {
             const uint32_t kk = L.skey[s];
             const uint32_t b = (radix_hash(kk) >> shift) & bmask;
@@ -176,14 +227,14 @@ __device__ __forceinline__ bool scatter_range(const ScatterLds& L, const uint32_
                 L.cv[slot] = L.sval[s];
             }
         }
-        __syncthreads();
-        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
-            L.cur[b] += L.nw[b];
-            L.ccnt[b] = L.ccnt[b] + L.cnt[b] - L.nw[b];
-        }
-        __syncthreads();
     }
-    // the remaining carries (partial chunks at the range ends)
+    __syncthreads();
+    // retire the last tile, then the remaining carries (partial chunks at the range ends)
+    {
+        uint32_t* cprev = L.cnt0 + (par ? 0 : nbk);
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) scatter_retire(L, cprev, b);
+    }
+    __syncthreads();
     for (int idx = threadIdx.x; idx < (nbk << cs_log2); idx += BLOCK) {
         const int b = idx >> cs_log2;
         const uint32_t i = (uint32_t)idx & (cs - 1u);
@@ -283,16 +334,20 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
         if (lane == 0)
             *s_base = total ? atomicAdd((unsigned long long*)count, (unsigned long long)total) : 0ull;
     }
+    // survivors' values, all in flight before the stores
+    uint32_t vv[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++)
+        if (pass[it]) vv[it] = sval[tile + (uint64_t)it * BLOCK + threadIdx.x];
     __syncthreads();
     bool bad = false;
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         if (pass[it]) {
-            const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
             const uint64_t pos = *s_base + gcnt[it * NW + wid] + rank[it];
             if (pos < out_cap) {
                 okey[pos] = k[it];
-                oval[pos] = sval[i];
+                oval[pos] = vv[it];
             } else {
                 bad = true;
             }
@@ -423,15 +478,41 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         if (total == 0) continue;  // uniform
         if (threadIdx.x == 0)
             L.s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
+        // while the reservation is in flight: the first match of every hit key and
+        // the values of both sides, all loads of the round issued together
+        uint32_t q0[PITEMS], pv[PITEMS], bv[PITEMS];
+#pragma unroll
+        for (int it = 0; it < PITEMS; it++) {
+            q0[it] = 0u;
+            if (hitmask & (1u << it)) {
+                const uint32_t st = info[it] & 0xFFFFu;
+                uint32_t q = st;
+                while (L.hkey[q] != kk[it]) q++;
+                q0[it] = q;
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < PITEMS; it++) {
+            if (hitmask & (1u << it)) {
+                pv[it] = pvp[base + (uint32_t)it * BLOCK + threadIdx.x];
+                bv[it] = bvp[L.hidx[q0[it]]];
+            }
+        }
         __syncthreads();
         uint64_t w = L.s_base + pre;
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
             if (hitmask & (1u << it)) {
-                const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
-                const uint32_t pval = pvp[idx];
-                const uint32_t st = info[it] & 0xFFFFu, en = st + (info[it] >> 16);
-                for (uint32_t q = st; q < en; q++) {
+                const uint32_t pval = pv[it];
+                if (w < out_cap) {
+                    o_key[w] = kk[it];
+                    o_rval[w] = build_r ? bv[it] : pval;
+                    o_sval[w] = build_r ? pval : bv[it];
+                }
+                w++;
+                // further matches of the same key (duplicate build keys)
+                const uint32_t en = (info[it] & 0xFFFFu) + (info[it] >> 16);
+                for (uint32_t q = q0[it] + 1; q < en; q++) {
                     if (L.hkey[q] == kk[it]) {
                         if (w < out_cap) {
                             const uint32_t bval = bvp[L.hidx[q]];

@@ -408,6 +408,7 @@ int  bj_server_release(bj_server *s, uint64_t id);
 /* Blocks until every submitted request has completed. */
 int  bj_server_drain(bj_server *s);
 int  bj_server_stats_get(bj_server *s, bj_server_stats *out);
+/* Waits for the requests on the device, drops queued ones, frees everything. */
 int  bj_server_destroy(bj_server *s);
 
 #ifdef __cplusplus
