@@ -112,7 +112,9 @@ int resolve(const bj_ctx* c, const bj_join_spec& s, JoinCfg& o)
     o.bpk = bpk;
     if (bpk > 0.0) {
         if (s.num_hashes > 8) return fail(BJ_ERR_INVALID, "num_hashes must be in [1, 8]");
-        o.k = s.num_hashes > 0 ? s.num_hashes : (int)std::lround(bpk * std::log(2.0));
+        // all k bits of a key share one 64-bit word: past about 4 bits the false-positive
+        // rate barely moves while every extra bit costs instructions per key
+        o.k = s.num_hashes > 0 ? s.num_hashes : std::min(4, (int)std::lround(bpk * std::log(2.0)));
         o.k = std::max(1, std::min(8, o.k));
         const double num_bits = std::max(256.0, (double)nr * bpk);
         const uint64_t bytes = (uint64_t)std::ceil(num_bits / 8.0);
@@ -583,16 +585,16 @@ int gen_tune_data(bj_ctx* c, TuneData& d, int m, uint64_t nr, uint64_t ns, doubl
 }
 
 // Best-of-3 device time of one configuration on the tuning batch.
-int time_config(bj_ctx* c, const TuneData& d, int bits, int tile, double bpk, double& ms_out)
+int time_config(bj_ctx* c, TuneData& d, int bits, int tile, double bpk, double& ms_out)
 {
     std::vector<bj_join_spec> specs(d.m);
     for (int j = 0; j < d.m; j++) {
         bj_join_spec& s = specs[j];
         bj_join_spec_default(&s);
-        s.r_key = P<uint32_t>(const_cast<DevBuf&>(d.rk)) + j * d.nr;
-        s.r_val = P<uint32_t>(const_cast<DevBuf&>(d.rv)) + j * d.nr;
-        s.s_key = P<uint32_t>(const_cast<DevBuf&>(d.sk)) + j * d.ns;
-        s.s_val = P<uint32_t>(const_cast<DevBuf&>(d.sv)) + j * d.ns;
+        s.r_key = P<uint32_t>(d.rk) + j * d.nr;
+        s.r_val = P<uint32_t>(d.rv) + j * d.nr;
+        s.s_key = P<uint32_t>(d.sk) + j * d.ns;
+        s.s_val = P<uint32_t>(d.sv) + j * d.ns;
         s.r_size = (int64_t)d.nr;
         s.s_size = (int64_t)d.ns;
         s.radix_bits = bits;

@@ -286,7 +286,7 @@ typedef struct bj_join_spec {
     int32_t radix_bits;              /* 1..11; -1 = autotuner / size rule        */
     int32_t tile;                    /* 4096 or 8192; 0 = autotuner / 8192       */
     double  bf_bits_per_key;         /* > 0: filter on; 0: off; < 0: autotuner   */
-    int32_t num_hashes;              /* 1..8; -1 = round(bits_per_key * ln 2)    */
+    int32_t num_hashes;              /* 1..8; -1 = min(round(bits_per_key ln 2), 4) */
     double  answer_scale;            /* row capacity = max(R,S) * max(1, scale)  */
     double  match_hint;              /* expected fraction of S with a match, used
                                         only to pick the autotuner's table entry;

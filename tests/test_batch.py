@@ -33,8 +33,8 @@ def test_plan_size_rule_and_overrides():
     assert cfg[3].radix_bits == 3 and cfg[3].bf_bits_per_key == 0.0 and cfg[3].bf_k == 0
     assert cfg[4].bf_bits_per_key == 12.0 and cfg[4].bf_k == 4
     assert cfg[4].capacity == 2 * (4 << 20)
-    # unknown selectivity: filter on at 10 bits per key, k = round(10 ln 2) = 7
-    assert cfg[0].bf_bits_per_key == 10.0 and cfg[0].bf_k == 7
+    # unknown selectivity: filter on at 10 bits per key, k = min(round(10 ln 2), 4) = 4
+    assert cfg[0].bf_bits_per_key == 10.0 and cfg[0].bf_k == 4
     # an empty side needs no filter
     assert cfg[5].bf_k == 0 and cfg[6].bf_k == 0
     assert cfg[1].capacity == 32 << 20
