@@ -351,12 +351,16 @@ def run_serve(args, opts: dict) -> dict:
 
 
 def load_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of a kernel slot, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes recorded in profiles/traffic_<workload>.json (each counter scaled by what it
+    reads for a plain device copy of known size); None when not recorded."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get(kernel)
+    e = d.get(kernel)
+    return int(e["bytes"]) if isinstance(e, dict) and "bytes" in e else None
 
 
 def main() -> int:

@@ -23,7 +23,8 @@ using namespace bj;
 
 namespace {
 
-constexpr int B_MAX_BITS = 11;                   // one scatter pass per join
+constexpr int B_MAX_BITS = 15;                   // joint histogram of a chunk in LDS: 2^15 x 4 B
+constexpr int B_ONE_PASS_BITS = 9;               // up to 512-way: one scatter pass
 constexpr uint64_t B_MAX_TUPLES = 0x7FFFFFFFull;
 constexpr uint32_t B_SLICE = 65536;
 constexpr uint32_t B_MAX_SLICES = 16;
@@ -33,6 +34,7 @@ constexpr int B_MAX_JOINS = 1 << 20;
 
 struct JoinCfg {
     int bits = 1;
+    int b1 = 1;  // first-pass bits (== bits: one pass)
     int tile = 8192;
     double bpk = 0.0;
     int k = 0;
@@ -95,13 +97,15 @@ int resolve(const bj_ctx* c, const bj_join_spec& s, JoinCfg& o)
     if (s.tile != 0 && s.tile != 4096 && s.tile != 8192) return fail(BJ_ERR_INVALID, "tile must be 4096 or 8192");
     o.tile = s.tile ? s.tile : (t0 ? t0->tile : 8192);
     if (s.radix_bits > 0) {
-        if (s.radix_bits > B_MAX_BITS) return fail(BJ_ERR_SIZE, "batched joins use at most 11 radix bits");
+        if (s.radix_bits > B_MAX_BITS) return fail(BJ_ERR_SIZE, "batched joins use at most 15 radix bits");
         o.bits = s.radix_bits;
     } else if (t0 && t0->tile == o.tile) {
         o.bits = t0->radix_bits;
     } else {
         o.bits = rule_bits(m, o.tile);
     }
+
+    o.b1 = o.bits <= B_ONE_PASS_BITS ? o.bits : (o.bits + 1) / 2;
 
     double bpk = s.bf_bits_per_key;
     if (bpk < 0.0) {
@@ -135,10 +139,21 @@ int resolve(const bj_ctx* c, const bj_join_spec& s, JoinCfg& o)
     return BJ_OK;
 }
 
-uint32_t chunk_len(uint64_t n)
+// Histogram / scatter chunk of a relation: at least 32K tuples, at most 512
+// chunks, and at most 2^21 per-chunk histogram entries per relation.
+uint32_t chunk_len(uint64_t n, int bits)
 {
-    const uint64_t per = (n + 511) / 512;
+    const uint64_t max_chunks = std::min<uint64_t>(512, std::max<uint64_t>(16, (1ull << 21) >> bits));
+    const uint64_t per = (n + max_chunks - 1) / max_chunks;
     return (uint32_t)std::max<uint64_t>(32768, (per + 4095) / 4096 * 4096);
+}
+
+// Second-pass workgroups per first-level bucket: about 64K tuples each.
+uint32_t pass2_groups(uint64_t n, int b1, uint64_t chunks)
+{
+    const uint64_t per_bucket = n >> b1;
+    const uint64_t w = std::max<uint64_t>(1, per_bucket / 65536);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(64, chunks), w));
 }
 
 // The layout of a batch: per-join descriptors in internal order (tile 8192
@@ -149,8 +164,9 @@ struct Layout {
     std::vector<JoinCfg> cfg;
     std::vector<uint32_t> px; // PX_N x (n + 1)
     uint32_t nt_r = 0, nt_s = 0, nseg = 0, ncol_r = 0, ncol_s = 0, nsb = 0, nsb_a = 0, max_wps = 1;
-    int max_bits = 1;
-    bool any_bf = false;
+    uint32_t nt_r2 = 0, nt_s2 = 0;
+    int max_bits = 1, max_b1 = 1, max_b2 = 0;
+    bool any_bf = false, any_two = false;
     uint64_t tot_r = 0, tot_s = 0, tot_sp = 0, part_n = 0, f_blocks = 0, out_n = 0, item_cap = 16;
 };
 
@@ -172,7 +188,7 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
     L.cfg.assign(n, JoinCfg{});
     L.px.assign((size_t)PX_N * (n + 1), 0u);
     auto PX = [&](int which, int i) -> uint32_t& { return L.px[(size_t)which * (n + 1) + i]; };
-    uint64_t nt_r = 0, nt_s = 0, nseg = 0, ncol_r = 0, ncol_s = 0, nsb = 0, nsb_a = 0;
+    uint64_t nt_r = 0, nt_s = 0, nseg = 0, ncol_r = 0, ncol_s = 0, nsb = 0, nsb_a = 0, nt_r2 = 0, nt_s2 = 0;
     for (int i = 0; i < n; i++) {
         const bj_join_spec& s = specs[L.order[i]];
         const JoinCfg& g = cfg[L.order[i]];
@@ -185,6 +201,7 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
         J.nr = (uint32_t)s.r_size;
         J.ns = (uint32_t)s.s_size;
         J.bits = g.bits;
+        J.b1 = g.b1;
         J.bf = g.bpk > 0.0 ? 1 : 0;
         J.geom = g.geom;
         J.tile = (uint32_t)g.tile;
@@ -193,9 +210,20 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
         PX(PX_SB, i) = (uint32_t)nsb;
         nsb += nbins;
         if (g.tile == 8192) nsb_a = nsb;
-        J.r_ch = chunk_len(J.nr);
-        J.s_ch = chunk_len(J.ns);
+        J.r_ch = chunk_len(J.nr, g.bits);
+        J.s_ch = chunk_len(J.ns, g.bits);
         const uint64_t cr = (J.nr + J.r_ch - 1) / J.r_ch, cs = (J.ns + J.s_ch - 1) / J.s_ch;
+        PX(PX_R2, i) = (uint32_t)nt_r2;
+        PX(PX_S2, i) = (uint32_t)nt_s2;
+        if (g.b1 < g.bits) {
+            J.w2_r = pass2_groups(J.nr, g.b1, cr);
+            J.w2_s = pass2_groups(J.ns, g.b1, cs);
+            nt_r2 += (uint64_t)J.w2_r << g.b1;
+            nt_s2 += (uint64_t)J.w2_s << g.b1;
+            L.any_two = true;
+            L.max_b2 = std::max(L.max_b2, g.bits - g.b1);
+        }
+        L.max_b1 = std::max(L.max_b1, g.b1);
         PX(PX_RTASK, i) = (uint32_t)nt_r;
         nt_r += cr;
         PX(PX_STASK, i) = (uint32_t)nt_s;
@@ -234,7 +262,9 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
     PX(PX_RCOL, n) = (uint32_t)ncol_r;
     PX(PX_SCOL, n) = (uint32_t)ncol_s;
     PX(PX_SB, n) = (uint32_t)nsb;
-    if (std::max({nt_r, nt_s, nseg, ncol_r, ncol_s, nsb}) > 0x7FFFFFFFull)
+    PX(PX_R2, n) = (uint32_t)nt_r2;
+    PX(PX_S2, n) = (uint32_t)nt_s2;
+    if (std::max({nt_r, nt_s, nseg, ncol_r, ncol_s, nsb, nt_r2, nt_s2}) > 0x7FFFFFFFull)
         return fail(BJ_ERR_SIZE, "batch too large for one launch per phase");
     L.nt_r = (uint32_t)nt_r;
     L.nt_s = (uint32_t)nt_s;
@@ -243,6 +273,8 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
     L.ncol_s = (uint32_t)ncol_s;
     L.nsb = (uint32_t)nsb;
     L.nsb_a = (uint32_t)nsb_a;
+    L.nt_r2 = (uint32_t)nt_r2;
+    L.nt_s2 = (uint32_t)nt_s2;
     return BJ_OK;
 }
 
@@ -253,7 +285,7 @@ struct bj_batch {
     int n = 0;
     Layout L;
     DevBuf dJ, dpx;
-    DevBuf r_key, r_val, s_key, s_val, sp_key, sp_val, part, cur_rel, tot_r, tot_s, start_r, start_s;
+    DevBuf r_key, r_val, s_key, s_val, sp_key, sp_val, tmp_key, tmp_val, part, cur_rel, tot_r, tot_s, start_r, start_s;
     DevBuf filter, plan_cnt, item_start, partials, items, ctr, out_key, out_rval, out_sval;
     uint64_t* h_ctr = nullptr;  // pinned: jc[n][JC_N], then err
     hipGraph_t graph = nullptr;
@@ -281,7 +313,10 @@ int batch_workspace(bj_batch* b, const Layout& L, int n, uint64_t* total_bytes)
         (rc = need(&bj_batch::dpx, L.px.size() * 4)) || (rc = need(&bj_batch::r_key, L.tot_r * 4)) ||
         (rc = need(&bj_batch::r_val, L.tot_r * 4)) || (rc = need(&bj_batch::s_key, L.tot_s * 4)) ||
         (rc = need(&bj_batch::s_val, L.tot_s * 4)) || (rc = need(&bj_batch::sp_key, L.tot_sp * 4)) ||
-        (rc = need(&bj_batch::sp_val, L.tot_sp * 4)) || (rc = need(&bj_batch::part, L.part_n * 4)) ||
+        (rc = need(&bj_batch::sp_val, L.tot_sp * 4)) ||
+        (rc = need(&bj_batch::tmp_key, L.any_two ? std::max(L.tot_r, L.tot_s) * 4 : 0)) ||
+        (rc = need(&bj_batch::tmp_val, L.any_two ? std::max(L.tot_r, L.tot_s) * 4 : 0)) ||
+        (rc = need(&bj_batch::part, L.part_n * 4)) ||
         (rc = need(&bj_batch::cur_rel, L.part_n * 4)) || (rc = need(&bj_batch::tot_r, nsb * 4)) ||
         (rc = need(&bj_batch::tot_s, nsb * 4)) || (rc = need(&bj_batch::start_r, nsb * 8)) ||
         (rc = need(&bj_batch::start_s, nsb * 8)) || (rc = need(&bj_batch::filter, L.f_blocks * 8)) ||
@@ -306,6 +341,8 @@ int enqueue(bj_batch* b, hipStream_t st)
     B.px = P<uint32_t>(b->dpx);
     B.n = n;
     B.max_bits = L.max_bits;
+    B.max_b1 = L.max_b1;
+    B.max_b2 = L.max_b2;
     uint64_t* ctr = P<uint64_t>(b->ctr);
     uint64_t* jc = ctr;
     uint64_t* work = ctr + (size_t)n * JC_N;
@@ -320,7 +357,11 @@ int enqueue(bj_batch* b, hipStream_t st)
     BJ_HIP(launch_batch_binscan(B, 0, P<uint32_t>(b->tot_r), P<uint64_t>(b->start_r), st));
     BJ_HIP(launch_batch_scatter(B, 0, L.nt_r, nullptr, nullptr, jc, P<uint32_t>(b->cur_rel),
                                 P<uint64_t>(b->start_r), P<uint32_t>(b->r_key), P<uint32_t>(b->r_val),
-                                b->r_key.bytes / 4, err, st));
+                                b->r_key.bytes / 4, P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
+                                b->tmp_key.bytes / 4, err, st));
+    BJ_HIP(launch_batch_scatter2(B, 0, L.nt_r2, P<uint32_t>(b->cur_rel), P<uint64_t>(b->start_r),
+                                 P<uint32_t>(b->tot_r), P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
+                                 P<uint32_t>(b->r_key), P<uint32_t>(b->r_val), b->r_key.bytes / 4, err, st));
     // Bloom joins: filter from the partitioned R, then S -> S'
     if (L.any_bf) {
         BJ_HIP(launch_batch_bloom_build(B, L.nseg, L.max_wps, P<uint32_t>(b->r_key), P<uint64_t>(b->start_r),
@@ -335,7 +376,11 @@ int enqueue(bj_batch* b, hipStream_t st)
     BJ_HIP(launch_batch_binscan(B, 1, P<uint32_t>(b->tot_s), P<uint64_t>(b->start_s), st));
     BJ_HIP(launch_batch_scatter(B, 1, L.nt_s, P<uint32_t>(b->sp_key), P<uint32_t>(b->sp_val), jc,
                                 P<uint32_t>(b->cur_rel), P<uint64_t>(b->start_s), P<uint32_t>(b->s_key),
-                                P<uint32_t>(b->s_val), b->s_key.bytes / 4, err, st));
+                                P<uint32_t>(b->s_val), b->s_key.bytes / 4, P<uint32_t>(b->tmp_key),
+                                P<uint32_t>(b->tmp_val), b->tmp_key.bytes / 4, err, st));
+    BJ_HIP(launch_batch_scatter2(B, 1, L.nt_s2, P<uint32_t>(b->cur_rel), P<uint64_t>(b->start_s),
+                                 P<uint32_t>(b->tot_s), P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
+                                 P<uint32_t>(b->s_key), P<uint32_t>(b->s_val), b->s_key.bytes / 4, err, st));
     // vlsplit plan and the persistent build + probe, one launch per tile class
     BJ_HIP(launch_batch_plan(B, L.nsb, P<uint64_t>(b->start_r), P<uint32_t>(b->tot_r), P<uint64_t>(b->start_s),
                              P<uint32_t>(b->tot_s), B_SLICE, B_MAX_SLICES, P<uint32_t>(b->plan_cnt),
@@ -410,7 +455,8 @@ int bj_batch_destroy(bj_batch* b)
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
     DevBuf* bufs[] = {&b->dJ,      &b->dpx,     &b->r_key,   &b->r_val,    &b->s_key,      &b->s_val,
-                      &b->sp_key,  &b->sp_val,  &b->part,    &b->cur_rel,  &b->tot_r,      &b->tot_s,
+                      &b->sp_key,  &b->sp_val,  &b->tmp_key, &b->tmp_val,  &b->part,       &b->cur_rel,
+                      &b->tot_r,   &b->tot_s,
                       &b->start_r, &b->start_s, &b->filter,  &b->plan_cnt, &b->item_start, &b->partials,
                       &b->items,   &b->ctr,     &b->out_key, &b->out_rval, &b->out_sval};
     for (DevBuf* d : bufs) release(*d);

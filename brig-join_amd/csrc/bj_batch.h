@@ -4,8 +4,10 @@
 // launch over all joins: the workgroups of a launch are dealt out as tasks
 // (a histogram/scatter chunk, a filter segment, a column block) and each task
 // finds its join by a binary search over a per-join prefix of task counts.
-// Every join is partitioned in one level of its own radix bits into its region
-// of shared arenas; the probe is a persistent launch over the sub-buckets of
+// Every join is partitioned with its own radix bits into its region of shared
+// arenas, in one scatter pass (up to 9 bits) or two (up to 15: the first by the
+// top bits into a scratch arena at the final positions, the second within each
+// first-level bucket); the probe is a persistent launch over the sub-buckets of
 // all joins.
 #pragma once
 
@@ -24,12 +26,14 @@ struct BJoin {
     const uint32_t* sk;       // S keys / values (device)
     const uint32_t* sv;
     uint32_t nr, ns;          // sizes (< 2^31)
-    int bits;                 // radix bits, one level (1..11)
+    int bits;                 // radix bits (1..15)
+    int b1;                   // bits of the first scatter pass (== bits: one pass)
     int bf;                   // Bloom filter on
     BloomGeom geom;           // filter geometry (segments = top seg_bits radix bits)
     uint32_t sb_base;         // first global sub-bucket
     uint32_t r_ch, s_ch;      // chunk length of R / S histogram and scatter tasks
     uint32_t tile;            // LDS table capacity of the probe: 4096 or 8192
+    uint32_t w2_r, w2_s;      // second-pass workgroups per first-level bucket (R / S)
     uint64_t r_base, s_base;  // region offsets in the partitioned R / S arenas
     uint64_t sp_base;         // region offset in the S' staging arena (Bloom joins)
     uint64_t part_r, part_s;  // per-chunk histogram offsets in the part arena
@@ -41,13 +45,14 @@ struct BJoin {
 enum { JC_SPRIME = 0, JC_OUT = 1, JC_N = 2 };
 
 // Prefix arrays (n + 1 entries each, u32) that map a task index to its join.
-enum { PX_RTASK = 0, PX_STASK, PX_SEG, PX_RCOL, PX_SCOL, PX_SB, PX_N };
+enum { PX_RTASK = 0, PX_STASK, PX_SEG, PX_RCOL, PX_SCOL, PX_SB, PX_R2, PX_S2, PX_N };
 
 struct BatchDev {
     const BJoin* J;           // [n]
     const uint32_t* px;       // [PX_N][n + 1]
     int n;
     int max_bits;             // largest radix bits of the batch
+    int max_b1, max_b2;       // largest bits of the first / second scatter pass
 };
 
 hipError_t batch_kernels_init(int max_lds_bytes);
@@ -60,10 +65,18 @@ hipError_t launch_batch_colscan(const BatchDev& B, int side, uint32_t ncolblk, c
                                 uint32_t* cur_rel, uint32_t* tot, hipStream_t st);
 hipError_t launch_batch_binscan(const BatchDev& B, int side, const uint32_t* tot, uint64_t* start,
                                 hipStream_t st);
+// First pass: every join's chunks by its first-pass bits, to the final arena
+// (one-pass joins) or to the scratch arena tkey/tval at the final positions.
 hipError_t launch_batch_scatter(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* sp_key,
                                 const uint32_t* sp_val, const uint64_t* jc, const uint32_t* cur_rel,
                                 const uint64_t* start, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
-                                uint64_t* err, hipStream_t st);
+                                uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err,
+                                hipStream_t st);
+// Second pass of two-pass joins: scratch arena -> final arena.
+hipError_t launch_batch_scatter2(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* cur_rel,
+                                 const uint64_t* start, const uint32_t* cnt, const uint32_t* tkey,
+                                 const uint32_t* tval, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
+                                 uint64_t* err, hipStream_t st);
 hipError_t launch_batch_bloom_build(const BatchDev& B, uint32_t nseg, uint32_t max_wps, const uint32_t* rkey,
                                     const uint64_t* r_start, const uint32_t* r_cnt, uint64_t* filter,
                                     hipStream_t st);

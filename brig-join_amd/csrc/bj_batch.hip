@@ -157,6 +157,12 @@ __global__ __launch_bounds__(1024) void kb_binscan(BatchDev B, int side, const u
 }
 
 // ============================================================== scatter
+// First pass: task (join, chunk) scatters its chunk by the join's first-pass
+// bits b1.  The cursor of first-level bucket B is the start of its first
+// sub-bucket plus everything earlier chunks put into its sub-buckets,
+// start[B << b2] + sum_j cur_rel[c][(B << b2) + j].  One-pass joins (b1 == bits)
+// write the final arena, two-pass joins the scratch arena at the same
+// positions.
 template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void kb_scatter(BatchDev B, int side, const uint32_t* __restrict__ sp_key,
                                                     const uint32_t* __restrict__ sp_val,
@@ -164,7 +170,8 @@ __global__ __launch_bounds__(BLOCK) void kb_scatter(BatchDev B, int side, const 
                                                     const uint32_t* __restrict__ cur_rel,
                                                     const uint64_t* __restrict__ start, uint32_t* __restrict__ okey,
                                                     uint32_t* __restrict__ oval, uint64_t out_cap,
-                                                    uint64_t* __restrict__ err)
+                                                    uint32_t* __restrict__ tkey, uint32_t* __restrict__ tval,
+                                                    uint64_t tmp_cap, uint64_t* __restrict__ err)
 {
     extern __shared__ uint64_t lds64[];
     const uint32_t* px = px_of(B, side ? PX_STASK : PX_RTASK);
@@ -172,20 +179,79 @@ __global__ __launch_bounds__(BLOCK) void kb_scatter(BatchDev B, int side, const 
     const int b = find_join(px, B.n, t);
     const BJoin J = B.J[b];
     const uint32_t c = t - px[b];
-    const int nbk = 1 << J.bits;
+    const int b2 = J.bits - J.b1;
+    const int nbk = 1 << J.b1;
     const int cs_log2 = nbk <= 512 ? 4 : (nbk <= 1024 ? 3 : 2);  // scatter_carry
     const ScatterLds L = scatter_lds<BLOCK, ITEMS>(lds64, nbk, cs_log2);
-    const uint32_t* cr = cur_rel + (side ? J.part_s : J.part_r) + (size_t)c * nbk;
+    const uint32_t* cr = cur_rel + (side ? J.part_s : J.part_r) + ((size_t)c << J.bits);
     const uint64_t* st = start + J.sb_base;
-    for (int j = threadIdx.x; j < nbk; j += BLOCK) L.cur[j] = st[j] + cr[j];
+    for (int j = threadIdx.x; j < nbk; j += BLOCK) {
+        uint64_t cu = st[(size_t)j << b2];
+        for (int q = 0; q < (1 << b2); q++) cu += cr[((size_t)j << b2) + q];
+        L.cur[j] = cu;
+    }
     const uint32_t *key, *val;
     uint64_t n;
     side_src(J, b, side, sp_key, sp_val, jc, key, val, n);
     const uint64_t ch = side ? J.s_ch : J.r_ch;
     const uint64_t lo = min((uint64_t)c * ch, n), hi = min(lo + ch, n);
-    const bool bad = scatter_range<BLOCK, ITEMS>(L, key + lo, val + lo, (uint32_t)(hi - lo), nbk, 32 - J.bits,
-                                                 (uint32_t)nbk - 1u, cs_log2, okey, oval, out_cap);
+    const bool two = b2 > 0;
+    const bool bad = scatter_range<BLOCK, ITEMS>(L, key + lo, val + lo, (uint32_t)(hi - lo), nbk, 32 - J.b1,
+                                                 (uint32_t)nbk - 1u, cs_log2, two ? tkey : okey, two ? tval : oval,
+                                                 two ? tmp_cap : out_cap);
     if (bad) atomicOr((unsigned long long*)err, 16ull);
+}
+
+// Second pass of a two-pass join: task (join, first-level bucket B, group w)
+// takes what the first pass wrote into B from chunks [g0, g1) of group w
+// (w2 groups per bucket) and scatters it by the remaining b2 bits.  Its range
+// and its sub-bucket cursors come from cur_rel at chunks g0 and g1.
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void kb_scatter2(BatchDev B, int side, const uint32_t* __restrict__ cur_rel,
+                                                     const uint64_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ tkey,
+                                                     const uint32_t* __restrict__ tval, uint32_t* __restrict__ okey,
+                                                     uint32_t* __restrict__ oval, uint64_t out_cap,
+                                                     uint64_t* __restrict__ err)
+{
+    extern __shared__ uint64_t lds64[];
+    __shared__ unsigned long long s_lo, s_hi;
+    const uint32_t* px = px_of(B, side ? PX_S2 : PX_R2);
+    const uint32_t t = blockIdx.x;
+    const int b = find_join(px, B.n, t);
+    const BJoin J = B.J[b];
+    const uint32_t local = t - px[b];
+    const uint32_t W2 = side ? J.w2_s : J.w2_r;
+    const int bb = (int)(local / W2), w = (int)(local % W2);
+    const int b2 = J.bits - J.b1;
+    const int nbk = 1 << b2;
+    const int G = (int)side_chunks(J, side);
+    const int g0 = (int)(((int64_t)w * G) / W2), g1 = (int)(((int64_t)(w + 1) * G) / W2);
+    const int cs_log2 = nbk <= 512 ? 4 : (nbk <= 1024 ? 3 : 2);
+    const ScatterLds L = scatter_lds<BLOCK, ITEMS>(lds64, nbk, cs_log2);
+    const size_t f0 = (size_t)bb << b2;  // first sub-bucket of the bucket, within the join
+    const uint32_t* base = cur_rel + (side ? J.part_s : J.part_r);
+    const uint64_t* st = start + J.sb_base + f0;
+    const uint32_t* tt = cnt + J.sb_base + f0;
+    if (threadIdx.x == 0) {
+        s_lo = 0ull;
+        s_hi = 0ull;
+    }
+    __syncthreads();
+    // range: first-level cursors at chunks g0 and g1 (the bucket's end for G)
+    for (int j = threadIdx.x; j < nbk; j += BLOCK) {
+        const uint32_t a = g0 < G ? base[((size_t)g0 << J.bits) + f0 + j] : tt[j];
+        const uint32_t e = g1 < G ? base[((size_t)g1 << J.bits) + f0 + j] : tt[j];
+        if (a) atomicAdd(&s_lo, (unsigned long long)a);
+        if (e) atomicAdd(&s_hi, (unsigned long long)e);
+        L.cur[j] = st[j] + (g0 < G ? a : tt[j]);
+    }
+    __syncthreads();
+    const uint64_t lo = st[0] + s_lo, hi = st[0] + s_hi;
+    const bool bad = scatter_range<BLOCK, ITEMS>(L, tkey + lo, tval + lo, (uint32_t)(hi - lo), nbk, 32 - J.bits,
+                                                 (uint32_t)nbk - 1u, cs_log2, okey, oval, out_cap);
+    if (bad) atomicOr((unsigned long long*)err, 64ull);
 }
 
 // ============================================================== Bloom build
@@ -360,6 +426,7 @@ hipError_t batch_kernels_init(int max_lds_bytes)
     hipError_t e;
     if ((e = set_lds_b((const void*)kb_hist<BH_BLOCK>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds_b((const void*)kb_scatter<BS_BLOCK, BS_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds_b((const void*)kb_scatter2<BS_BLOCK, BS_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds_b((const void*)kb_bloom_build<1024>, max_lds_bytes)) != hipSuccess) return e;
     return hipSuccess;
 }
@@ -391,12 +458,25 @@ hipError_t launch_batch_binscan(const BatchDev& B, int side, const uint32_t* tot
 hipError_t launch_batch_scatter(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* sp_key,
                                 const uint32_t* sp_val, const uint64_t* jc, const uint32_t* cur_rel,
                                 const uint64_t* start, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
-                                uint64_t* err, hipStream_t st)
+                                uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err,
+                                hipStream_t st)
 {
     if (ntasks == 0) return hipSuccess;
-    const size_t lds = scatter_lds_bytes(1 << B.max_bits);
+    const size_t lds = scatter_lds_bytes(1 << B.max_b1);
     kb_scatter<BS_BLOCK, BS_ITEMS><<<ntasks, BS_BLOCK, lds, st>>>(B, side, sp_key, sp_val, jc, cur_rel, start,
-                                                                  okey, oval, out_cap, err);
+                                                                  okey, oval, out_cap, tkey, tval, tmp_cap, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_scatter2(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* cur_rel,
+                                 const uint64_t* start, const uint32_t* cnt, const uint32_t* tkey,
+                                 const uint32_t* tval, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
+                                 uint64_t* err, hipStream_t st)
+{
+    if (ntasks == 0) return hipSuccess;
+    const size_t lds = scatter_lds_bytes(1 << std::max(1, B.max_b2));
+    kb_scatter2<BS_BLOCK, BS_ITEMS><<<ntasks, BS_BLOCK, lds, st>>>(B, side, cur_rel, start, cnt, tkey, tval,
+                                                                   okey, oval, out_cap, err);
     return hipGetLastError();
 }
 

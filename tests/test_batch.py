@@ -26,9 +26,9 @@ def test_plan_size_rule_and_overrides():
              spec(1 << 20, 4 << 20, bf_bits_per_key=12.0, num_hashes=4, answer_scale=2.0),
              spec(0, 1000), spec(1000, 0)]
     cfg, ws = bj.batch_plan(specs)
-    # size rule: smallest bits with min(R,S) / 2^bits <= 0.6 * tile, at most 11
+    # size rule: smallest bits with min(R,S) / 2^bits <= 0.6 * tile, at most 15
     assert cfg[0].radix_bits == math.ceil(math.log2((1 << 16) / (0.6 * 8192)))
-    assert cfg[1].radix_bits == 11 and cfg[1].tile == 8192
+    assert cfg[1].radix_bits == 12 and cfg[1].tile == 8192
     assert cfg[2].tile == 4096 and cfg[2].radix_bits == 1
     assert cfg[3].radix_bits == 3 and cfg[3].bf_bits_per_key == 0.0 and cfg[3].bf_k == 0
     assert cfg[4].bf_bits_per_key == 12.0 and cfg[4].bf_k == 4
@@ -50,7 +50,7 @@ def test_plan_selectivity_hint_rule():
 
 
 @pytest.mark.parametrize("kw,code", [
-    (dict(tile=1000), -1), (dict(radix_bits=12), -5), (dict(num_hashes=9, bf_bits_per_key=8.0), -1),
+    (dict(tile=1000), -1), (dict(radix_bits=16), -5), (dict(num_hashes=9, bf_bits_per_key=8.0), -1),
 ])
 def test_plan_rejects_bad_specs(kw, code):
     with pytest.raises(bj.BjError) as e:
@@ -103,9 +103,11 @@ def _mixed_small_joins():
     rng = np.random.default_rng(11)
     joins = []
     # duplicates on both sides, assorted bits / tiles / filters
+    # radix bits above 9 take two scatter passes
     for i, kw in enumerate([dict(), dict(bf_bits_per_key=10.0), dict(radix_bits=1), dict(radix_bits=11),
                             dict(tile=4096, radix_bits=4), dict(bf_bits_per_key=4.0, num_hashes=2),
-                            dict(tile=4096, bf_bits_per_key=8.0), dict(radix_bits=7)]):
+                            dict(tile=4096, bf_bits_per_key=8.0), dict(radix_bits=7),
+                            dict(radix_bits=14, bf_bits_per_key=10.0), dict(radix_bits=10, tile=4096)]):
         nr, ns = int(rng.integers(1, 30000)), int(rng.integers(1, 60000))
         rk = rng.integers(0, 20000, nr, dtype=np.uint32)
         sk = rng.integers(0, 30000, ns, dtype=np.uint32)
