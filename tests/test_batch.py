@@ -115,8 +115,9 @@ def _mixed_small_joins():
                               np.arange(ns, dtype=np.uint32) * 3 + i, **kw))
     # special keys, empty sides, a heavy key beyond one LDS tile
     sp = np.array([0, 1, 0x80000000, 0xFFFFFFFF, 0x7FFFFFFF], np.uint32)
+    # 75 rows: more than max(|R|, |S|), so the capacity is raised
     joins.append(HostJoin(np.tile(sp, 3), np.arange(15, dtype=np.uint32), np.tile(sp, 5),
-                          np.arange(25, dtype=np.uint32)))
+                          np.arange(25, dtype=np.uint32), answer_scale=4.0))
     joins.append(HostJoin(np.zeros(0, np.uint32), np.zeros(0, np.uint32), sp, np.arange(5, dtype=np.uint32)))
     joins.append(HostJoin(sp, np.arange(5, dtype=np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
     hk = np.full(20000, 12345, np.uint32)
@@ -239,7 +240,7 @@ def test_autotune_and_tuned_batch(gpu_ctx):
     for e in table:
         assert e.kind in (0, 1) and e.ms > 0
         if e.kind == 0:
-            assert 1 <= e.radix_bits <= 11 and e.tile in (4096, 8192)
+            assert 1 <= e.radix_bits <= 15 and e.tile in (4096, 8192)
         else:
             assert e.bf_bits_per_key in (0.0, 6.0, 10.0, 14.0)
     # classes already tuned are kept
@@ -254,7 +255,7 @@ def test_autotune_and_tuned_batch(gpu_ctx):
     # the table round-trips, and bad entries are refused
     bj.tune_set(gpu_ctx, table)
     assert len(bj.tune_get(gpu_ctx)) == len(table)
-    bad = bj.BjTuneEntry(kind=0, lg=16, radix_bits=12, tile=8192)
+    bad = bj.BjTuneEntry(kind=0, lg=16, radix_bits=16, tile=8192)
     with pytest.raises(bj.BjError):
         bj.tune_set(gpu_ctx, [bad])
     bj.tune_set(gpu_ctx, [])
