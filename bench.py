@@ -374,6 +374,8 @@ def main() -> int:
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--bf-stage", type=int, default=None, help="override: 1 before, 2 after pass 1")
     ap.add_argument("--pass1-bits", type=int, default=None, help="override the first scatter pass bits")
+    ap.add_argument("--s-pass1-bits", type=int, default=None, help="override S's first scatter pass bits")
+    ap.add_argument("--scatter-wg", type=int, default=None, help="scatter workgroup size (512 or 1024)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE) and exit")
@@ -418,6 +420,10 @@ def main() -> int:
         params = dict(params, bf_stage=args.bf_stage)
     if args.pass1_bits is not None:
         params = dict(params, pass1_bits=args.pass1_bits)
+    if args.s_pass1_bits is not None:
+        params = dict(params, s_pass1_bits=args.s_pass1_bits)
+    if args.scatter_wg is not None:
+        params = dict(params, scatter_wg=args.scatter_wg)
     p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=1, **params)
     bloom = bool(params.get("use_bf"))
 
@@ -472,6 +478,7 @@ def main() -> int:
                    "zipf_s": zipf_s, **{k: v for k, v in params.items()},
                    "bits1": int(res.bits1), "bits2": int(res.bits2), "tile": int(res.tile),
                    "bf_stage": int(res.bf_stage), "pass1_bits": int(res.pass1_bits),
+                   "s_pass1_bits": int(res.s_pass1_bits),
                    "bf_seg_bits": int(res.bf_seg_bits), "bf_k": int(res.bf_k)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

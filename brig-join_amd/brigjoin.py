@@ -57,7 +57,8 @@ class BjParams(ctypes.Structure):
         ("answer_scale", ctypes.c_double), ("tile", ctypes.c_int32),
         ("inputs_on_device", ctypes.c_int32), ("collect_phase_times", ctypes.c_int32),
         ("s_chunks", ctypes.c_int32), ("bf_stage", ctypes.c_int32),
-        ("pass1_bits", ctypes.c_int32),
+        ("pass1_bits", ctypes.c_int32), ("s_pass1_bits", ctypes.c_int32),
+        ("scatter_wg", ctypes.c_int32),
     ]
 
 
@@ -72,7 +73,7 @@ class BjResult(ctypes.Structure):
         ("bits1", ctypes.c_int32), ("bits2", ctypes.c_int32), ("bf_seg_bits", ctypes.c_int32),
         ("bf_k", ctypes.c_int32), ("bf_words", ctypes.c_uint64), ("tile", ctypes.c_int32),
         ("heavy_items", ctypes.c_uint64), ("bf_stage", ctypes.c_int32),
-        ("pass1_bits", ctypes.c_int32),
+        ("pass1_bits", ctypes.c_int32), ("s_pass1_bits", ctypes.c_int32),
     ]
 
     def phases(self) -> dict:

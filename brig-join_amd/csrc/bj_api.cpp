@@ -114,7 +114,10 @@ constexpr int MAX_BITS = 15;                     // joint histogram in LDS: 2^15
 constexpr int MAX_LEVEL_BITS = 11;               // fan-out per scatter pass
 constexpr uint64_t SEG_TARGET_BYTES = 64 * 1024; // Bloom segment built in LDS
 constexpr uint64_t SEG_MAX_BYTES = 160 * 1024;
-constexpr uint64_t MALL_FILTER_BYTES = 128ull << 20;  // filters up to half the MALL probe well unpartitioned
+// A filter slice probed at random stays in one XCD's 4 MB L2 up to about this size:
+// a smaller filter is probed before S is partitioned (bf_stage 1), a larger one
+// slice by slice after S's first pass (bf_stage 2), with enough first-pass bits.
+constexpr uint64_t L2_SLICE_BYTES = 3584ull << 10;
 
 int choose_G(uint64_t n)
 {
@@ -232,8 +235,6 @@ int make_plan(const bj_params& p, Plan& pl)
     }
     if (pl.p1 > MAX_LEVEL_BITS || total - pl.p1 > MAX_LEVEL_BITS)
         return fail(BJ_ERR_SIZE, "a scatter pass exceeds 2048-way fan-out");
-    // enough second-pass workgroups to fill the chip: about 1024 in total
-    pl.W2 = std::max(1, std::min(64, 1024 >> pl.p1));
 
     if (pl.bf) {
         int seg_bits = std::min(seg_need, total);
@@ -246,12 +247,26 @@ int make_plan(const bj_params& p, Plan& pl)
         pl.geom.k = k;
         pl.bf_words = (wps << seg_bits) * 2;  // in 32-bit words
         if (p.bf_stage < 0 || p.bf_stage > 2) return fail(BJ_ERR_INVALID, "bf_stage must be 0, 1 or 2");
-        pl.bf_stage = p.bf_stage ? p.bf_stage : (pl.bf_words * 4 <= MALL_FILTER_BYTES ? 1 : 2);
+        pl.bf_stage = p.bf_stage ? p.bf_stage : (pl.bf_words * 4 <= L2_SLICE_BYTES ? 1 : 2);
     }
+    // first-pass bits of S
+    if (p.s_pass1_bits > 0) {
+        if (p.s_pass1_bits > total) return fail(BJ_ERR_INVALID, "s_pass1_bits exceeds the total radix bits");
+        pl.p1_s = p.s_pass1_bits;
+    } else if (pl.bf && pl.bf_stage == 2) {
+        const int need = ceil_log2((pl.bf_words * 4 + L2_SLICE_BYTES - 1) / L2_SLICE_BYTES);
+        pl.p1_s = std::max(pl.p1, std::min(std::min(total, 9), need));
+    } else {
+        pl.p1_s = pl.p1;
+    }
+    if (pl.p1_s > MAX_LEVEL_BITS || total - pl.p1_s > MAX_LEVEL_BITS)
+        return fail(BJ_ERR_SIZE, "a scatter pass of S exceeds 2048-way fan-out");
+    if (p.scatter_wg != 0 && p.scatter_wg != 512 && p.scatter_wg != 1024)
+        return fail(BJ_ERR_INVALID, "scatter_wg must be 0, 512 or 1024");
+    pl.scatter_wg = p.scatter_wg ? p.scatter_wg : 1024;
 
     pl.G_r = choose_G(pl.nr);
     pl.G_s = choose_G(pl.ns);
-    pl.W2 = std::min(pl.W2, std::min(pl.G_r, pl.G_s));
     const double scale = std::max(1.0, p.answer_scale);
     pl.cap = std::max<uint64_t>(1, (uint64_t)((double)std::max(pl.nr, pl.ns) * scale));
     // sum over sub-buckets of ceil(nb / tile) <= (nr + ns) / tile + 2^bits
@@ -260,11 +275,10 @@ int make_plan(const bj_params& p, Plan& pl)
 }
 
 // Radix partition of one relation into out_key/out_val with sub-bucket starts
-// in start[0 .. 2^bits], in one or two physical scatter passes (pl.p1 bits,
-// then the rest).  With l1_only, only the first pass runs and its output
-// (grouped by first-pass bucket, in the final positions' ranges) is left in
-// out_key/out_val.
-int run_partition(bj_ctx* c, const Plan& pl, const uint32_t* key, const uint32_t* val,
+// in start[0 .. 2^bits], in one or two physical scatter passes (p1 bits, then
+// the rest).  With l1_only, only the first pass runs and its output (grouped by
+// first-pass bucket, in the final positions' ranges) is left in out_key/out_val.
+int run_partition(bj_ctx* c, const Plan& pl, int p1, const uint32_t* key, const uint32_t* val,
                   uint64_t n_host, const uint64_t* n_dev, int G, uint32_t* out_key,
                   uint32_t* out_val, uint64_t out_cap, uint64_t* start, int k_hist, bool l1_only = false)
 {
@@ -273,7 +287,7 @@ int run_partition(bj_ctx* c, const Plan& pl, const uint32_t* key, const uint32_t
     auto end = [&](int slot) { return hipEventRecord(c->ev[EV_KE + slot], c->stream); };
     hipStream_t st = c->stream;
     const int nbins = 1 << pl.bits;
-    const int p1 = pl.p1, b2 = pl.bits - pl.p1;
+    const int b2 = pl.bits - p1;
     const int P1 = 1 << p1;
     uint64_t* err = P<uint64_t>(c->counters) + CTR_ERR;
     uint32_t* part = P<uint32_t>(c->part);
@@ -286,7 +300,8 @@ int run_partition(bj_ctx* c, const Plan& pl, const uint32_t* key, const uint32_t
     const uint32_t* l1h = b2 > 0 ? P<uint32_t>(c->l1h) : part;
     BJ_HIP(launch_l1_offsets(l1h, G, P1, b2, start, P<uint64_t>(c->l1off), st));
     const bool two_pass = b2 > 0 && !l1_only;
-    const int W2 = std::min(pl.W2, G);
+    // enough second-pass workgroups to fill the chip: about 1024 in total
+    const int W2 = std::min(std::max(1, std::min(64, 1024 >> p1)), G);
     if (two_pass) BJ_HIP(launch_l2_offsets(part, G, pl.bits, W2, start, P<uint64_t>(c->l2cur), st));
     BJ_HIP(end(k_hist + 1));
     uint32_t* k1 = two_pass ? P<uint32_t>(c->tmp_key) : out_key;
@@ -294,12 +309,12 @@ int run_partition(bj_ctx* c, const Plan& pl, const uint32_t* key, const uint32_t
     const uint64_t cap1 = two_pass ? c->tmp_key.bytes / 4 : out_cap;
     BJ_HIP(beg(k_hist + 2));
     BJ_HIP(launch_scatter(key, val, k1, v1, cap1, n_host, n_dev, G, 0, pl.bits, p1, W2,
-                          P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, st));
+                          P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, pl.scatter_wg, st));
     BJ_HIP(end(k_hist + 2));
     BJ_HIP(beg(k_hist + 3));
     if (two_pass)
         BJ_HIP(launch_scatter(k1, v1, out_key, out_val, out_cap, n_host, n_dev, G, 1, pl.bits, p1, W2,
-                              P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, st));
+                              P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, pl.scatter_wg, st));
     BJ_HIP(end(k_hist + 3));
     return BJ_OK;
 }
@@ -312,14 +327,15 @@ int workspace_items(const Plan& pl, bool with_r, bool with_out,
     const uint64_t nr = std::max<uint64_t>(pl.nr, 1), ns = std::max<uint64_t>(pl.ns, 1);
     const uint64_t nbins = 1ull << pl.bits;
     const uint64_t gmax = (uint64_t)(with_r ? std::max(pl.G_r, pl.G_s) : pl.G_s);
-    const uint64_t n_l1 = (1ull << pl.p1) * gmax;
+    const uint64_t n_l1 = (1ull << std::max(pl.p1, pl.p1_s)) * gmax;
+    const uint64_t w2max = (uint64_t)std::max(1, std::min(64, 1024 >> std::min(pl.p1, pl.p1_s)));
     if (with_r) {
         if ((rc = need(&bj_ctx::r_key, nr * 4)) || (rc = need(&bj_ctx::r_val, nr * 4))) return rc;
         if ((rc = need(&bj_ctx::r_start, (nbins + 1) * 8))) return rc;
         if (pl.bf && (rc = need(&bj_ctx::filter, pl.bf_words * 4))) return rc;
     }
     if ((rc = need(&bj_ctx::s_key, ns * 4)) || (rc = need(&bj_ctx::s_val, ns * 4))) return rc;
-    if (pl.p1 < pl.bits || (pl.bf && pl.bf_stage == 2)) {
+    if (pl.p1 < pl.bits || pl.p1_s < pl.bits || (pl.bf && pl.bf_stage == 2)) {
         const uint64_t t = with_r ? std::max(nr, ns) : ns;
         if ((rc = need(&bj_ctx::tmp_key, t * 4)) || (rc = need(&bj_ctx::tmp_val, t * 4))) return rc;
     }
@@ -333,7 +349,7 @@ int workspace_items(const Plan& pl, bool with_r, bool with_out,
     if ((rc = need(&bj_ctx::totals, nbins * 4))) return rc;
     if ((rc = need(&bj_ctx::l1h, n_l1 * 4))) return rc;
     if ((rc = need(&bj_ctx::l1off, n_l1 * 8))) return rc;
-    if ((rc = need(&bj_ctx::l2cur, (uint64_t)pl.W2 * nbins * 8))) return rc;
+    if ((rc = need(&bj_ctx::l2cur, w2max * nbins * 8))) return rc;
     if ((rc = need(&bj_ctx::plan_cnt, nbins * 4))) return rc;
     if ((rc = need(&bj_ctx::item_start, (nbins + 1) * 8))) return rc;
     if ((rc = need(&bj_ctx::s_start, (nbins + 1) * 8))) return rc;
@@ -373,7 +389,7 @@ float ev_ms(bj_ctx* c, int a, int b)
 int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* r_val)
 {
     int rc;
-    if ((rc = run_partition(c, pl, r_key, r_val, pl.nr, nullptr, pl.G_r, P<uint32_t>(c->r_key),
+    if ((rc = run_partition(c, pl, pl.p1, r_key, r_val, pl.nr, nullptr, pl.G_r, P<uint32_t>(c->r_key),
                             P<uint32_t>(c->r_val), c->r_key.bytes / 4, P<uint64_t>(c->r_start), BJ_K_HIST_R)))
         return rc;
     if (pl.bf) {
@@ -402,28 +418,29 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
                                           P<uint32_t>(c->sp_val), c->sp_key.bytes / 4, ctr + CTR_SPRIME,
                                           ctr + CTR_ERR, c->num_cu, st));
             BJ_HIP(end(BJ_K_BF_FILTER));
-            if ((rc = run_partition(c, pl, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns, ctr + CTR_SPRIME,
+            if ((rc = run_partition(c, pl, pl.p1_s, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns,
+                                    ctr + CTR_SPRIME,
                                     pl.G_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), c->s_key.bytes / 4,
                                     P<uint64_t>(c->s_start), BJ_K_HIST_S)))
                 return rc;
         } else {
             // first-level pass of S, then filter each bucket against its L2-resident
             // filter slice while placing the survivors into their sub-buckets
-            if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->tmp_key),
+            if ((rc = run_partition(c, pl, pl.p1_s, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->tmp_key),
                                     P<uint32_t>(c->tmp_val), c->tmp_key.bytes / 4, P<uint64_t>(c->s_start),
                                     BJ_K_HIST_S, true)))
                 return rc;
             BJ_HIP(beg(BJ_K_BF_FILTER));
             BJ_HIP(launch_zero_u32(P<uint32_t>(c->s_fill), 1ull << pl.bits, st));
             BJ_HIP(launch_filter_l1(P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val), P<uint64_t>(c->s_start),
-                                    pl.bits, pl.bits - pl.p1, R.filter, pl.geom, P<uint32_t>(c->s_key),
+                                    pl.bits, pl.bits - pl.p1_s, R.filter, pl.geom, P<uint32_t>(c->s_key),
                                     P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint32_t>(c->s_fill),
                                     ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
             s_cnt = P<uint32_t>(c->s_fill);
             BJ_HIP(end(BJ_K_BF_FILTER));
         }
     } else {
-        if ((rc = run_partition(c, pl, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->s_key),
+        if ((rc = run_partition(c, pl, pl.p1_s, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->s_key),
                                 P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint64_t>(c->s_start),
                                 BJ_K_HIST_S)))
             return rc;
@@ -487,6 +504,8 @@ void bj_params_default(bj_params* p)
     p->s_chunks = 1;
     p->bf_stage = 0;
     p->pass1_bits = -1;
+    p->s_pass1_bits = -1;
+    p->scatter_wg = 0;
 }
 
 int bj_create(bj_ctx** out, int device)
@@ -657,6 +676,7 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
     out->heavy_items = h[CTR_N];
     out->bf_stage = pl.bf ? pl.bf_stage : 0;
     out->pass1_bits = pl.p1;
+    out->s_pass1_bits = pl.p1_s;
     if (p->collect_phase_times) {
         double* k = out->kernel_ms;
         for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = ev_ms(c, EV_KB + i, EV_KE + i);
@@ -688,7 +708,7 @@ int bj_partition(bj_ctx* c, const bj_params* p, const uint32_t* key, const uint3
     hipStream_t st = c->stream;
     uint64_t* ctr = P<uint64_t>(c->counters);
     BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
-    if ((rc = run_partition(c, pl, key, val, pl.nr, nullptr, pl.G_r, out_key, out_val, pl.nr, out_start,
+    if ((rc = run_partition(c, pl, pl.p1, key, val, pl.nr, nullptr, pl.G_r, out_key, out_val, pl.nr, out_start,
                             BJ_K_HIST_R)))
         return rc;
     BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));

@@ -49,8 +49,9 @@ enum { EV_H2D0 = 0, EV_H2D1, EV_KB, EV_KE = EV_KB + BJ_NUM_KERNELS, EV_N = EV_KE
 struct Plan {
     uint64_t nr = 0, ns = 0;
     int bits = 1, bits1 = 1, bits2 = 0;  // logical levels (partition_size, second level)
-    int p1 = 1;                          // bits of the first physical scatter pass
-    int W2 = 1;                          // second-pass workgroups per first-pass bucket
+    int p1 = 1;                          // bits of the first physical scatter pass (R)
+    int p1_s = 1;                        // the same for S
+    int scatter_wg = 1024;               // scatter workgroup size
     int G_r = 1, G_s = 1;
     int tile = 8192;
     uint32_t slice = 65536;

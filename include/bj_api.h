@@ -110,6 +110,10 @@ typedef struct bj_params {
                                        The resulting sub-bucket layout does not depend
                                        on it; partition_size is the logical first
                                        level, whose buckets are unions of sub-buckets  */
+    int32_t s_pass1_bits;           /* the same for S; -1 = auto (with bf_stage 2: enough
+                                       bits for each first-pass bucket's filter slice
+                                       to fit an XCD's L2, else pass1_bits)            */
+    int32_t scatter_wg;             /* scatter workgroup size, 512 or 1024; 0 = auto   */
 } bj_params;
 
 /* ---- join result ------------------------------------------------------- */
@@ -133,6 +137,7 @@ typedef struct bj_result {
     uint64_t heavy_items;     /* vlsplit work items for oversized sub-buckets        */
     int32_t  bf_stage;        /* 1 or 2 as in bj_params (0 = Bloom off)              */
     int32_t  pass1_bits;      /* radix bits of the first physical scatter pass       */
+    int32_t  s_pass1_bits;    /* the same for S                                      */
 } bj_result;
 
 /* ---- the plan a join would use (no device needed) ---------------------- */

@@ -67,6 +67,8 @@ VARIANTS = [
     dict(radix_bits=4, tile=4096),
     dict(partition_size=64, second_partition_size=16, use_bf=10),
     dict(radix_bits=13, use_bf=1, bf_bits_per_key=4.0, num_hashes=2),
+    dict(use_bf=1, bf_stage=2, radix_bits=12, s_pass1_bits=8, scatter_wg=512),
+    dict(radix_bits=10, pass1_bits=3, s_pass1_bits=8, scatter_wg=512),
 ]
 
 
@@ -105,6 +107,8 @@ def test_generator_matches_oracle(gpu_ctx):
     (2_000_000, 300_000, 1.0, dict(tile=4096)),
     (1_000_000, 1_000_000, 0.1, dict(radix_bits=2)),           # vlsplit: chunks x slices
     (77_777, 1_234_567, 0.3, dict(partition_size=32, radix_bits=9)),
+    (1 << 21, 3 << 20, 0.1, dict(use_bf=1, radix_bits=14, bf_stage=2, s_pass1_bits=9, scatter_wg=512)),
+    (1 << 20, 1 << 21, 0.1, dict(use_bf=1, radix_bits=13, bf_stage=2, s_pass1_bits=4)),
 ])
 def test_generated_parity(gpu_ctx, n_r, n_s, ratio, kw):
     m = O.match_count(n_s, ratio)
