@@ -443,6 +443,14 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
         const uint64_t chunk = (hi - lo + W - 1) / W;
         const uint64_t clo = min(lo + (uint64_t)j * chunk, hi);
         const uint64_t chi = min(clo + chunk, hi);
+        // keys are streamed once (non-temporal, so the filter slice keeps the L2);
+        // the next tile's keys load while the current tile is processed
+        uint32_t nk[ITEMS];
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const uint64_t i = clo + (uint64_t)it * BLOCK + threadIdx.x;
+            nk[it] = (i < chi) ? __builtin_nontemporal_load(&ikey[i]) : 0u;
+        }
         for (uint64_t tile = clo; tile < chi; tile += T, par ^= 1u) {
             const int tn = (int)min((uint64_t)T, chi - tile);
             uint32_t* cnt = cnt0 + (par ? P2 : 0);
@@ -452,8 +460,12 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
 #pragma unroll
             for (int it = 0; it < ITEMS; it++) {
                 const int s = it * BLOCK + threadIdx.x;
-                // streamed once: non-temporal, so the filter slice keeps the L2
-                k[it] = (s < tn) ? __builtin_nontemporal_load(&ikey[tile + s]) : 0u;
+                k[it] = (s < tn) ? nk[it] : 0u;
+            }
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++) {
+                const uint64_t i = tile + T + (uint64_t)it * BLOCK + threadIdx.x;
+                if (i < chi) nk[it] = __builtin_nontemporal_load(&ikey[i]);
             }
             // all filter words of the tile in flight before any is tested
             uint64_t fw[ITEMS];
