@@ -56,12 +56,6 @@ __device__ __forceinline__ void side_src(const BJoin& J, int b, int side, const 
     }
 }
 
-// log2 of the carry chunk (tuples) of an nbk-way scatter, as scatter_carry
-__device__ __forceinline__ int log2_carry(int nbk)
-{
-    return nbk <= 128 ? 5 : (nbk <= 512 ? 4 : (nbk <= 1024 ? 3 : 2));
-}
-
 __device__ __forceinline__ uint32_t side_chunks(const BJoin& J, int side)
 {
     const uint64_t n = side ? J.ns : J.nr;
@@ -187,7 +181,7 @@ __global__ __launch_bounds__(BLOCK) void kb_scatter(BatchDev B, int side, const 
     const uint32_t c = t - px[b];
     const int b2 = J.bits - J.b1;
     const int nbk = 1 << J.b1;
-    const int cs_log2 = log2_carry(nbk);
+    const int cs_log2 = nbk <= 512 ? 4 : (nbk <= 1024 ? 3 : 2);  // scatter_carry
     const ScatterLds L = scatter_lds<BLOCK, ITEMS>(lds64, nbk, cs_log2);
     const uint32_t* cr = cur_rel + (side ? J.part_s : J.part_r) + ((size_t)c << J.bits);
     const uint64_t* st = start + J.sb_base;
@@ -234,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void kb_scatter2(BatchDev B, int side, const
     const int nbk = 1 << b2;
     const int G = (int)side_chunks(J, side);
     const int g0 = (int)(((int64_t)w * G) / W2), g1 = (int)(((int64_t)(w + 1) * G) / W2);
-    const int cs_log2 = log2_carry(nbk);
+    const int cs_log2 = nbk <= 512 ? 4 : (nbk <= 1024 ? 3 : 2);
     const ScatterLds L = scatter_lds<BLOCK, ITEMS>(lds64, nbk, cs_log2);
     const size_t f0 = (size_t)bb << b2;  // first sub-bucket of the bucket, within the join
     const uint32_t* base = cur_rel + (side ? J.part_s : J.part_r);

@@ -784,9 +784,9 @@ hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* 
 }
 
 // Write-combining geometry: tile T = SC_BLOCK * SC_ITEMS tuples, carry chunk
-// cs tuples: one 128 B L2 line per array up to 128 buckets, then 64 B up to
-// 512 buckets, then 32 B / 16 B so the carry stays at 64 KB of LDS.
-int scatter_carry(int nbk) { return nbk <= 128 ? 32 : (nbk <= 512 ? 16 : (nbk <= 1024 ? 8 : 4)); }
+// cs tuples (64 B up to 512 buckets, then 32 B / 16 B so the carry stays at
+// 64 KB of LDS).
+int scatter_carry(int nbk) { return nbk <= 512 ? 16 : (nbk <= 1024 ? 8 : 4); }
 
 static int log2i(int x)
 {
