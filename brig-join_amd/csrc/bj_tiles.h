@@ -359,21 +359,22 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
 
 // ---------------------------------------------------------------- build + probe of one work item
 // The build side's chunk (bn <= CAP tuples) is counting-sorted into an LDS
-// table by the hash bits below the radix bits: binfo[t] packs (start, count)
-// of bucket t, hkey[] holds the keys bucket by bucket and hidx[] their row in
-// the chunk (6 bytes per build tuple).  A probe key reads one binfo word, then
-// scans a short contiguous run of keys: no pointer chasing.  The probe side
-// (pn keys) streams through in rounds of BLOCK*PITEMS keys (the next round is
-// loaded while the current one is processed).  Each round counts its matches,
+// table by the hash bits below the radix bits, about two buckets per build
+// tuple: bst[t] is the first slot of bucket t (bst[nbk] = bn), hkey[] holds the
+// keys bucket by bucket and hidx[] their row in the chunk (8 bytes of LDS per
+// build tuple).  A probe key reads two bucket starts, then scans a short
+// contiguous run of keys: no pointer chasing.  The probe side (pn keys)
+// streams through in rounds of BLOCK*PITEMS keys (the next round is loaded
+// while the current one is processed).  Each round counts its matches,
 // reserves its rows with one global atomic after a block scan, then writes
-// them.  Both values of a row are read only for matches: the build side's from
-// its (cache-resident) chunk, the probe side's from its slice.  Rows past
+// them.  Both values of a row are read only for matches: the build side's
+// from its (cache-resident) chunk, the probe side's from its slice.  Rows past
 // out_cap are counted, not written.
 template <int BLOCK, int CAP>
 struct ProbeLds {
-    uint32_t hkey[CAP];
+    uint32_t hkey[CAP];      // keys by bucket (bucket counts while the table is built)
     uint16_t hidx[CAP];
-    uint32_t binfo[CAP / 2];
+    uint16_t bst[CAP + 2];   // bucket starts, CAP buckets at most
     uint32_t wtmp[BLOCK / WAVE + 1];
     unsigned long long s_base;
 };
@@ -387,20 +388,20 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
                                            uint32_t* __restrict__ o_key, uint32_t* __restrict__ o_rval,
                                            uint32_t* __restrict__ o_sval)
 {
-    constexpr int NBMAX = CAP / 2;
     constexpr int ROUND = BLOCK * PITEMS;
     constexpr int BPT = CAP / BLOCK;  // build tuples per thread
-    static_assert(CAP % BLOCK == 0 && CAP <= 65535, "table indices are 16-bit");
+    static_assert(CAP % BLOCK == 0 && CAP <= 32768, "bucket and rank pack into 16 bits each");
 
-    // table geometry: 2^tb buckets, about 1..2 build tuples per bucket
+    // table geometry: 2^tb buckets, about two per build tuple
     int tb = 0;
-    while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= NBMAX) tb++;
+    while ((1 << tb) < 2 * bn && (1 << tb) < CAP) tb++;
     const int nbk = 1 << tb;
     const int tshift = 32 - tb;
-    for (int i = threadIdx.x; i < nbk; i += BLOCK) L.binfo[i] = 0u;
+    uint32_t* cnt = L.hkey;  // bucket counts until the keys are placed
+    for (int i = threadIdx.x; i < nbk; i += BLOCK) cnt[i] = 0u;
     __syncthreads();
     // build: rank each tuple in its bucket, scan the counts, place;
-    // bucket (< 2^12) and rank (< 2^16) of each build tuple, packed
+    // bucket (< 2^15) and rank (< 2^16) of each build tuple, packed
     uint32_t bkey[BPT], btr[BPT];
 #pragma unroll
     for (int u = 0; u < BPT; u++) {
@@ -412,30 +413,30 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         const int i = u * BLOCK + threadIdx.x;
         if (i < bn) {
             const uint32_t t = tb ? ((radix_hash(bkey[u]) << bits) >> tshift) : 0u;
-            btr[u] = (atomicAdd(&L.binfo[t], 1u) << 16) | t;
+            btr[u] = (atomicAdd(&cnt[t], 1u) << 16) | t;
         }
     }
     __syncthreads();
     {
-        // exclusive scan of the counts; binfo[t] = start | count << 16
+        // exclusive scan of the counts into the bucket starts
         const int per_t = (nbk + BLOCK - 1) / BLOCK;
         const int lo_t = min((int)threadIdx.x * per_t, nbk), hi_t = min(lo_t + per_t, nbk);
         uint32_t sum = 0;
-        for (int t = lo_t; t < hi_t; t++) sum += L.binfo[t];
+        for (int t = lo_t; t < hi_t; t++) sum += cnt[t];
         uint32_t tot;
         uint32_t ex = block_exclusive_scan<BLOCK, uint32_t>(sum, L.wtmp, &tot);
         for (int t = lo_t; t < hi_t; t++) {
-            const uint32_t c = L.binfo[t];
-            L.binfo[t] = ex | (c << 16);
-            ex += c;
+            L.bst[t] = (uint16_t)ex;
+            ex += cnt[t];
         }
+        if (threadIdx.x == 0) L.bst[nbk] = (uint16_t)bn;
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < BPT; u++) {
         const int i = u * BLOCK + threadIdx.x;
         if (i < bn) {
-            const uint32_t q = (L.binfo[btr[u] & 0xFFFFu] & 0xFFFFu) + (btr[u] >> 16);
+            const uint32_t q = (uint32_t)L.bst[btr[u] & 0xFFFFu] + (btr[u] >> 16);
             L.hkey[q] = bkey[u];
             L.hidx[q] = (uint16_t)i;
         }
@@ -449,7 +450,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         nk[it] = (idx < pn) ? pkp[idx] : 0u;
     }
     for (uint32_t base = 0; base < pn; base += ROUND) {
-        uint32_t kk[PITEMS], info[PITEMS];
+        uint32_t kk[PITEMS], qs[PITEMS], qe[PITEMS];
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
         // prefetch the next round
@@ -461,15 +462,16 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
             const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
-            info[it] = (idx < pn) ? L.binfo[tb ? ((radix_hash(kk[it]) << bits) >> tshift) : 0u] : 0u;
+            const uint32_t t = tb ? ((radix_hash(kk[it]) << bits) >> tshift) : 0u;
+            qs[it] = (idx < pn) ? (uint32_t)L.bst[t] : 0u;
+            qe[it] = (idx < pn) ? (uint32_t)L.bst[t + 1] : 0u;
         }
         // count
         uint32_t c = 0, hitmask = 0;
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
-            const uint32_t st = info[it] & 0xFFFFu, en = st + (info[it] >> 16);
             uint32_t m = 0;
-            for (uint32_t q = st; q < en; q++) m += (L.hkey[q] == kk[it]);
+            for (uint32_t q = qs[it]; q < qe[it]; q++) m += (L.hkey[q] == kk[it]);
             c += m;
             hitmask |= (m ? 1u : 0u) << it;
         }
@@ -485,8 +487,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         for (int it = 0; it < PITEMS; it++) {
             q0[it] = 0u;
             if (hitmask & (1u << it)) {
-                const uint32_t st = info[it] & 0xFFFFu;
-                uint32_t q = st;
+                uint32_t q = qs[it];
                 while (L.hkey[q] != kk[it]) q++;
                 q0[it] = q;
             }
@@ -511,8 +512,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
                 }
                 w++;
                 // further matches of the same key (duplicate build keys)
-                const uint32_t en = (info[it] & 0xFFFFu) + (info[it] >> 16);
-                for (uint32_t q = q0[it] + 1; q < en; q++) {
+                for (uint32_t q = q0[it] + 1; q < qe[it]; q++) {
                     if (L.hkey[q] == kk[it]) {
                         if (w < out_cap) {
                             const uint32_t bval = bvp[L.hidx[q]];
