@@ -359,8 +359,8 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
 
 // ---------------------------------------------------------------- build + probe of one work item
 // The build side's chunk (bn <= CAP tuples) is counting-sorted into an LDS
-// table by the hash bits below the radix bits, about two buckets per build
-// tuple: bst[t] is the first slot of bucket t (bst[nbk] = bn), hkey[] holds the
+// table by the hash bits below the radix bits (up to two buckets per build
+// tuple): bst[t] is the first slot of bucket t (bst[nbk] = bn), hkey[] holds the
 // keys bucket by bucket and hidx[] their row in the chunk (8 bytes of LDS per
 // build tuple).  A probe key reads two bucket starts, then scans a short
 // contiguous run of keys: no pointer chasing.  The probe side (pn keys)
@@ -392,9 +392,15 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
     constexpr int BPT = CAP / BLOCK;  // build tuples per thread
     static_assert(CAP % BLOCK == 0 && CAP <= 32768, "bucket and rank pack into 16 bits each");
 
-    // table geometry: 2^tb buckets, about two per build tuple
+    // table geometry: 2^tb buckets, about two per build tuple when the probe
+    // side is much longer than the build side (shorter runs per probe key), else
+    // one to two build tuples per bucket (less table to clear and scan)
     int tb = 0;
-    while ((1 << tb) < 2 * bn && (1 << tb) < CAP) tb++;
+    if (pn >= 8u * (uint32_t)bn) {
+        while ((1 << tb) < 2 * bn && (1 << tb) < CAP) tb++;
+    } else {
+        while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= CAP / 2) tb++;
+    }
     const int nbk = 1 << tb;
     const int tshift = 32 - tb;
     uint32_t* cnt = L.hkey;  // bucket counts until the keys are placed
