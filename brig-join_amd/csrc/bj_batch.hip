@@ -164,7 +164,7 @@ __global__ __launch_bounds__(1024) void kb_binscan(BatchDev B, int side, const u
 // write the final arena, two-pass joins the scratch arena at the same
 // positions.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void kb_scatter(BatchDev B, int side, const uint32_t* __restrict__ sp_key,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void kb_scatter(BatchDev B, int side, const uint32_t* __restrict__ sp_key,
                                                     const uint32_t* __restrict__ sp_val,
                                                     const uint64_t* __restrict__ jc,
                                                     const uint32_t* __restrict__ cur_rel,
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(BLOCK) void kb_scatter(BatchDev B, int side, const 
 // (w2 groups per bucket) and scatters it by the remaining b2 bits.  Its range
 // and its sub-bucket cursors come from cur_rel at chunks g0 and g1.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void kb_scatter2(BatchDev B, int side, const uint32_t* __restrict__ cur_rel,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void kb_scatter2(BatchDev B, int side, const uint32_t* __restrict__ cur_rel,
                                                      const uint64_t* __restrict__ start,
                                                      const uint32_t* __restrict__ cnt,
                                                      const uint32_t* __restrict__ tkey,

@@ -290,7 +290,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t* __rest
 //       (h >> (32 - bits)) & (2^(bits-p1) - 1); cur[b] = l2cur[w][(b1 << (bits-p1)) + b].
 // Writes are bounds-checked against out_cap and flag err on violation.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_scatter(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 1024 ? 8 : 6, 8))) void k_scatter(
     const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ ival,
     uint32_t* __restrict__ okey, uint32_t* __restrict__ oval, uint64_t out_cap,
     uint64_t n_host, const uint64_t* __restrict__ n_dev, int G, int mode, int bits, int p1, int W2,
