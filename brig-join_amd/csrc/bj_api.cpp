@@ -119,8 +119,16 @@ constexpr uint64_t SEG_MAX_BYTES = 160 * 1024;
 // slice by slice after S's first pass (bf_stage 2), with enough first-pass bits.
 constexpr uint64_t L2_SLICE_BYTES = 3584ull << 10;
 
+// Relations of at least this many tuples are scattered by one workgroup per CU
+// (one chunk each): with several 128-way write-combining workgroups per CU the
+// HBM write streams interleave more and a 640M-tuple pass runs 10-15% slower,
+// while smaller relations need the extra workgroups to hide latency.
+constexpr uint64_t SOLO_TUPLES = 32ull << 20;
+constexpr int PLAN_CUS = 256;  // CUs of an MI355X: only the work split depends on it
+
 int choose_G(uint64_t n)
 {
+    if (n >= SOLO_TUPLES) return PLAN_CUS;
     uint64_t g = (n + 32767) / 32768;
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(512, g));
 }
@@ -267,6 +275,8 @@ int make_plan(const bj_params& p, Plan& pl)
 
     pl.G_r = choose_G(pl.nr);
     pl.G_s = choose_G(pl.ns);
+    pl.solo_r = pl.nr >= SOLO_TUPLES;
+    pl.solo_s = pl.ns >= SOLO_TUPLES;
     const double scale = std::max(1.0, p.answer_scale);
     pl.cap = std::max<uint64_t>(1, (uint64_t)((double)std::max(pl.nr, pl.ns) * scale));
     // sum over sub-buckets of ceil(nb / tile) <= (nr + ns) / tile + 2^bits
@@ -279,9 +289,11 @@ int make_plan(const bj_params& p, Plan& pl)
 // the rest).  With l1_only, only the first pass runs and its output (grouped by
 // first-pass bucket, in the final positions' ranges) is left in out_key/out_val.
 int run_partition(bj_ctx* c, const Plan& pl, int p1, const uint32_t* key, const uint32_t* val,
-                  uint64_t n_host, const uint64_t* n_dev, int G, uint32_t* out_key,
+                  uint64_t n_host, const uint64_t* n_dev, int G, bool solo, uint32_t* out_key,
                   uint32_t* out_val, uint64_t out_cap, uint64_t* start, int k_hist, bool l1_only = false)
 {
+    // more than half of the CU's LDS: one scatter workgroup per CU
+    const size_t min_lds = solo ? (size_t)c->max_lds / 2 + 1024 : 0;
     // k_hist is the BJ_K_HIST_* slot; the three slots after it follow in order
     auto beg = [&](int slot) { return hipEventRecord(c->ev[EV_KB + slot], c->stream); };
     auto end = [&](int slot) { return hipEventRecord(c->ev[EV_KE + slot], c->stream); };
@@ -309,12 +321,12 @@ int run_partition(bj_ctx* c, const Plan& pl, int p1, const uint32_t* key, const 
     const uint64_t cap1 = two_pass ? c->tmp_key.bytes / 4 : out_cap;
     BJ_HIP(beg(k_hist + 2));
     BJ_HIP(launch_scatter(key, val, k1, v1, cap1, n_host, n_dev, G, 0, pl.bits, p1, W2,
-                          P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, pl.scatter_wg, st));
+                          P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, pl.scatter_wg, min_lds, st));
     BJ_HIP(end(k_hist + 2));
     BJ_HIP(beg(k_hist + 3));
     if (two_pass)
         BJ_HIP(launch_scatter(k1, v1, out_key, out_val, out_cap, n_host, n_dev, G, 1, pl.bits, p1, W2,
-                              P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, pl.scatter_wg, st));
+                              P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, pl.scatter_wg, min_lds, st));
     BJ_HIP(end(k_hist + 3));
     return BJ_OK;
 }
@@ -389,7 +401,7 @@ float ev_ms(bj_ctx* c, int a, int b)
 int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* r_val)
 {
     int rc;
-    if ((rc = run_partition(c, pl, pl.p1, r_key, r_val, pl.nr, nullptr, pl.G_r, P<uint32_t>(c->r_key),
+    if ((rc = run_partition(c, pl, pl.p1, r_key, r_val, pl.nr, nullptr, pl.G_r, pl.solo_r, P<uint32_t>(c->r_key),
                             P<uint32_t>(c->r_val), c->r_key.bytes / 4, P<uint64_t>(c->r_start), BJ_K_HIST_R)))
         return rc;
     if (pl.bf) {
@@ -420,13 +432,13 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
             BJ_HIP(end(BJ_K_BF_FILTER));
             if ((rc = run_partition(c, pl, pl.p1_s, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns,
                                     ctr + CTR_SPRIME,
-                                    pl.G_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), c->s_key.bytes / 4,
+                                    pl.G_s, pl.solo_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), c->s_key.bytes / 4,
                                     P<uint64_t>(c->s_start), BJ_K_HIST_S)))
                 return rc;
         } else {
             // first-level pass of S, then filter each bucket against its L2-resident
             // filter slice while placing the survivors into their sub-buckets
-            if ((rc = run_partition(c, pl, pl.p1_s, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->tmp_key),
+            if ((rc = run_partition(c, pl, pl.p1_s, s_key, s_val, pl.ns, nullptr, pl.G_s, pl.solo_s, P<uint32_t>(c->tmp_key),
                                     P<uint32_t>(c->tmp_val), c->tmp_key.bytes / 4, P<uint64_t>(c->s_start),
                                     BJ_K_HIST_S, true)))
                 return rc;
@@ -440,7 +452,7 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
             BJ_HIP(end(BJ_K_BF_FILTER));
         }
     } else {
-        if ((rc = run_partition(c, pl, pl.p1_s, s_key, s_val, pl.ns, nullptr, pl.G_s, P<uint32_t>(c->s_key),
+        if ((rc = run_partition(c, pl, pl.p1_s, s_key, s_val, pl.ns, nullptr, pl.G_s, pl.solo_s, P<uint32_t>(c->s_key),
                                 P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint64_t>(c->s_start),
                                 BJ_K_HIST_S)))
             return rc;
@@ -708,7 +720,7 @@ int bj_partition(bj_ctx* c, const bj_params* p, const uint32_t* key, const uint3
     hipStream_t st = c->stream;
     uint64_t* ctr = P<uint64_t>(c->counters);
     BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
-    if ((rc = run_partition(c, pl, pl.p1, key, val, pl.nr, nullptr, pl.G_r, out_key, out_val, pl.nr, out_start,
+    if ((rc = run_partition(c, pl, pl.p1, key, val, pl.nr, nullptr, pl.G_r, pl.solo_r, out_key, out_val, pl.nr, out_start,
                             BJ_K_HIST_R)))
         return rc;
     BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));

@@ -52,7 +52,8 @@ struct Plan {
     int p1 = 1;                          // bits of the first physical scatter pass (R)
     int p1_s = 1;                        // the same for S
     int scatter_wg = 1024;               // scatter workgroup size
-    int G_r = 1, G_s = 1;
+    int G_r = 1, G_s = 1;                // histogram / first-pass scatter workgroups
+    bool solo_r = false, solo_s = false; // one scatter workgroup per CU (large relations)
     int tile = 8192;
     uint32_t slice = 65536;
     uint32_t max_slices = 16;

@@ -39,11 +39,12 @@ hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* 
 int scatter_carry(int nbk);
 size_t scatter_lds_bytes(int nbk);
 // pass 1 (mode 0): p1-bit buckets; pass 2 (mode 1): the remaining bits, W2
-// workgroups per pass-1 bucket with cursors l2cur[w][sb]
+// workgroups per pass-1 bucket with cursors l2cur[w][sb].  The launch reserves at
+// least min_lds bytes of LDS per workgroup (a way to cap workgroups per CU).
 hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* okey, uint32_t* oval,
                           uint64_t out_cap, uint64_t n_host, const uint64_t* n_dev, int G, int mode,
                           int bits, int p1, int W2, const uint64_t* l1off, const uint64_t* sb_start,
-                          const uint64_t* l2cur, uint64_t* err, int wg, hipStream_t st);
+                          const uint64_t* l2cur, uint64_t* err, int wg, size_t min_lds, hipStream_t st);
 hipError_t launch_l2_offsets(const uint32_t* part, int G, int bits, int W2, const uint64_t* sb_start,
                              uint64_t* l2cur, hipStream_t st);
 hipError_t launch_bloom_build(const uint32_t* rkey, const uint64_t* r_start, int bits,

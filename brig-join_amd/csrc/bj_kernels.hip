@@ -813,18 +813,19 @@ hipError_t launch_l2_offsets(const uint32_t* part, int G, int bits, int W2, cons
 hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* okey, uint32_t* oval,
                           uint64_t out_cap, uint64_t n_host, const uint64_t* n_dev, int G, int mode,
                           int bits, int p1, int W2, const uint64_t* l1off, const uint64_t* sb_start,
-                          const uint64_t* l2cur, uint64_t* err, int wg, hipStream_t st)
+                          const uint64_t* l2cur, uint64_t* err, int wg, size_t min_lds, hipStream_t st)
 {
     const int nbk = (mode == 0) ? (1 << p1) : (1 << (bits - p1));
     const int grid = (mode == 0) ? G : (1 << p1) * W2;
     const int cs = log2i(scatter_carry(nbk));
+    const size_t lds = std::max(scatter_lds_bytes(nbk), min_lds);
     if (wg == 512) {
         // same tile (4096 tuples), half the threads: more workgroups per CU
-        k_scatter<SC_BLOCK / 2, SC_ITEMS * 2><<<grid, SC_BLOCK / 2, scatter_lds_bytes(nbk), st>>>(
+        k_scatter<SC_BLOCK / 2, SC_ITEMS * 2><<<grid, SC_BLOCK / 2, lds, st>>>(
             ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, l2cur,
             err);
     } else {
-        k_scatter<SC_BLOCK, SC_ITEMS><<<grid, SC_BLOCK, scatter_lds_bytes(nbk), st>>>(
+        k_scatter<SC_BLOCK, SC_ITEMS><<<grid, SC_BLOCK, lds, st>>>(
             ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, l2cur,
             err);
     }
