@@ -376,6 +376,7 @@ def main() -> int:
     ap.add_argument("--pass1-bits", type=int, default=None, help="override the first scatter pass bits")
     ap.add_argument("--s-pass1-bits", type=int, default=None, help="override S's first scatter pass bits")
     ap.add_argument("--scatter-wg", type=int, default=None, help="scatter workgroup size (512 or 1024)")
+    ap.add_argument("--tile", type=int, default=None, help="probe LDS table capacity (4096 or 8192)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE) and exit")
@@ -424,6 +425,8 @@ def main() -> int:
         params = dict(params, s_pass1_bits=args.s_pass1_bits)
     if args.scatter_wg is not None:
         params = dict(params, scatter_wg=args.scatter_wg)
+    if args.tile is not None:
+        params = dict(params, tile=args.tile)
     p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=1, **params)
     bloom = bool(params.get("use_bf"))
 

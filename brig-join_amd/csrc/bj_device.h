@@ -79,24 +79,37 @@ __host__ __device__ __forceinline__ uint64_t bloom_block(uint32_t key, const Blo
 }
 
 // The k bit positions are 6-bit fields of a second multiplicative mix (fields
-// taken from the top, xor-folded so the low fields see the high key bits).
+// taken from the top, xor-folded so the low fields see the high key bits):
+// field i < 5 is bits [26 - 6i, 32 - 6i) of h2, field i >= 5 the same of h3.
+// k is the same for every lane, so the loops below are scalar loops of k
+// iterations (no work for absent fields).
 __host__ __device__ __forceinline__ uint64_t bloom_mask(uint32_t key, int k)
 {
     uint32_t h2 = (key ^ 0x5BD1E995u) * 0xC2B2AE3Du;
     h2 ^= h2 >> 15;
-    uint32_t h3 = 0;
+    uint64_t m = 0;
+    const int k1 = k < 5 ? k : 5;
+#pragma unroll 1
+    for (int i = 0; i < k1; i++) m |= 1ull << ((h2 >> (26 - 6 * i)) & 63u);
     if (k > 5) {
-        h3 = h2 * 0x27D4EB2Fu;
+        uint32_t h3 = h2 * 0x27D4EB2Fu;
         h3 ^= h3 >> 15;
+#pragma unroll 1
+        for (int i = 5; i < k; i++) m |= 1ull << ((h3 >> (26 - 6 * (i - 5))) & 63u);
     }
+    return m;
+}
+
+// The same mask for a k fixed at compile time (k <= 5): straight-line code.
+template <int K>
+__device__ __forceinline__ uint64_t bloom_mask_k(uint32_t key)
+{
+    static_assert(K >= 1 && K <= 5, "fields of h2 only");
+    uint32_t h2 = (key ^ 0x5BD1E995u) * 0xC2B2AE3Du;
+    h2 ^= h2 >> 15;
     uint64_t m = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        if (i < k) {
-            const uint32_t bit = (i < 5) ? ((h2 >> (26 - 6 * i)) & 63u) : ((h3 >> (26 - 6 * (i - 5))) & 63u);
-            m |= 1ull << bit;
-        }
-    }
+    for (int i = 0; i < K; i++) m |= 1ull << ((h2 >> (26 - 6 * i)) & 63u);
     return m;
 }
 

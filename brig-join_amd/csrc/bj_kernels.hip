@@ -289,8 +289,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t* __rest
 //       bucket b1 written by pass-1 groups [w*G/W2, (w+1)*G/W2); bucket =
 //       (h >> (32 - bits)) & (2^(bits-p1) - 1); cur[b] = l2cur[w][(b1 << (bits-p1)) + b].
 // Writes are bounds-checked against out_cap and flag err on violation.
+// Occupancy targets: two 1024-thread workgroups per CU for the 4096-tuple tile,
+// one for the 8192-tuple tile (launched one per CU), three 512-thread ones.
+constexpr int sc_min_waves(int block, int items) { return block == 1024 ? (items == 4 ? 8 : 4) : 6; }
+
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 1024 ? 8 : 6, 8))) void k_scatter(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sc_min_waves(BLOCK, ITEMS), 8))) void k_scatter(
     const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ ival,
     uint32_t* __restrict__ okey, uint32_t* __restrict__ oval, uint64_t out_cap,
     uint64_t n_host, const uint64_t* __restrict__ n_dev, int G, int mode, int bits, int p1, int W2,
@@ -402,7 +406,8 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_prefilter(
 // (tile, sub-bucket) and written, compacted, into their sub-bucket's region of
 // the S layout (sub-bucket sb gets positions sb_start[sb] + [0, s_fill[sb])),
 // so no second pass over S is needed.  Values are read only for survivors.
-template <int BLOCK, int ITEMS>
+// K: the filter's bits per key when fixed at compile time (3 or 4), 0 = g.k.
+template <int BLOCK, int ITEMS, int K>
 __global__ __launch_bounds__(BLOCK) void k_filter_l1(
     const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ ival,
     const uint64_t* __restrict__ sb_start, int bits, int bits2, const uint64_t* __restrict__ filter,
@@ -444,7 +449,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
         const uint64_t clo = min(lo + (uint64_t)j * chunk, hi);
         const uint64_t chi = min(clo + chunk, hi);
         // keys are streamed once (non-temporal, so the filter slice keeps the L2);
-        // the next tile's keys load while the current tile is processed
+        // the next tile's keys load while the current tile is written out
         uint32_t nk[ITEMS];
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
@@ -462,22 +467,17 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
                 const int s = it * BLOCK + threadIdx.x;
                 k[it] = (s < tn) ? nk[it] : 0u;
             }
-#pragma unroll
-            for (int it = 0; it < ITEMS; it++) {
-                const uint64_t i = tile + T + (uint64_t)it * BLOCK + threadIdx.x;
-                if (i < chi) nk[it] = __builtin_nontemporal_load(&ikey[i]);
-            }
-            // all filter words of the tile in flight before any is tested
+            // all filter words of the tile in flight before any is tested.  The
+            // loads are unconditional (a lane past the tile reads the word of key
+            // 0): loads under a branch, with addresses from earlier loads, would
+            // each wait for all loads before them.
             uint64_t fw[ITEMS];
 #pragma unroll
-            for (int it = 0; it < ITEMS; it++) {
-                const int s = it * BLOCK + threadIdx.x;
-                fw[it] = (s < tn) ? filter[bloom_block(k[it], g)] : 0ull;
-            }
+            for (int it = 0; it < ITEMS; it++) fw[it] = filter[bloom_block(k[it], g)];
 #pragma unroll
             for (int it = 0; it < ITEMS; it++) {
                 const int s = it * BLOCK + threadIdx.x;
-                const uint64_t m = bloom_mask(k[it], g.k);
+                const uint64_t m = K ? bloom_mask_k<(K ? K : 1)>(k[it]) : bloom_mask(k[it], g.k);
                 const bool ok = s < tn && (fw[it] & m) == m;
                 pass[it] = ok;
                 if (ok) {
@@ -519,6 +519,13 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
 #pragma unroll
             for (int it = 0; it < ITEMS; it++)
                 if (pass[it]) vv[it] = __builtin_nontemporal_load(&ival[tile + it * BLOCK + threadIdx.x]);
+            // the next tile's keys, issued last so that no wait of this tile covers
+            // them (clamped rather than skipped past the end, as above)
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++) {
+                const uint64_t i = min(tile + T + (uint64_t)it * BLOCK + threadIdx.x, chi - 1);
+                nk[it] = __builtin_nontemporal_load(&ikey[i]);
+            }
             __syncthreads();
 #pragma unroll
             for (int it = 0; it < ITEMS; it++) {
@@ -707,6 +714,8 @@ hipError_t launch_gen_s(uint32_t* key, uint32_t* val, uint64_t n_s, uint64_t n_r
 constexpr int HIST_BLOCK = 1024;
 constexpr int SC_BLOCK = 1024;
 constexpr int SC_ITEMS = 4;
+constexpr int SC_BIG_ITEMS = 8;  // 8192-tuple tiles for one-workgroup-per-CU launches
+static size_t g_max_lds = 64 * 1024;
 constexpr int FL_BLOCK = 512;
 constexpr int FL_ITEMS = 8;
 
@@ -727,8 +736,12 @@ hipError_t kernels_init(int max_lds_bytes)
     if ((e = set_lds((const void*)k_hist<HIST_BLOCK>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds((const void*)k_scatter<SC_BLOCK, SC_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds((const void*)k_scatter<SC_BLOCK / 2, SC_ITEMS * 2>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds((const void*)k_scatter<SC_BLOCK, SC_BIG_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
+    g_max_lds = (size_t)max_lds_bytes;
     if ((e = set_lds((const void*)k_bloom_build<1024>, max_lds_bytes)) != hipSuccess) return e;
-    if ((e = set_lds((const void*)k_filter_l1<FL_BLOCK, FL_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds((const void*)k_filter_l1<FL_BLOCK, FL_ITEMS, 0>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds((const void*)k_filter_l1<FL_BLOCK, FL_ITEMS, 3>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds((const void*)k_filter_l1<FL_BLOCK, FL_ITEMS, 4>, max_lds_bytes)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -795,12 +808,13 @@ static int log2i(int x)
     return l;
 }
 
-size_t scatter_lds_bytes(int nbk)
+static size_t scatter_lds_tile(int nbk, int tile)
 {
     const int cs = scatter_carry(nbk);
-    return (size_t)nbk * 8 + (size_t)nbk * cs * 8 + (size_t)SC_BLOCK * SC_ITEMS * 8 + (size_t)nbk * 20 +
-           (SC_BLOCK / WAVE + 1) * 4;
+    return (size_t)nbk * 8 + (size_t)nbk * cs * 8 + (size_t)tile * 8 + (size_t)nbk * 20 + (SC_BLOCK / WAVE + 1) * 4;
 }
+
+size_t scatter_lds_bytes(int nbk) { return scatter_lds_tile(nbk, SC_BLOCK * SC_ITEMS); }
 
 hipError_t launch_l2_offsets(const uint32_t* part, int G, int bits, int W2, const uint64_t* sb_start,
                              uint64_t* l2cur, hipStream_t st)
@@ -819,7 +833,13 @@ hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* 
     const int grid = (mode == 0) ? G : (1 << p1) * W2;
     const int cs = log2i(scatter_carry(nbk));
     const size_t lds = std::max(scatter_lds_bytes(nbk), min_lds);
-    if (wg == 512) {
+    const size_t lds_big = std::max(scatter_lds_tile(nbk, SC_BLOCK * SC_BIG_ITEMS), min_lds);
+    if (wg != 512 && min_lds > 0 && lds_big <= g_max_lds) {
+        // one workgroup per CU: twice the tile, twice the loads in flight
+        k_scatter<SC_BLOCK, SC_BIG_ITEMS><<<grid, SC_BLOCK, lds_big, st>>>(
+            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, l2cur,
+            err);
+    } else if (wg == 512) {
         // same tile (4096 tuples), half the threads: more workgroups per CU
         k_scatter<SC_BLOCK / 2, SC_ITEMS * 2><<<grid, SC_BLOCK / 2, lds, st>>>(
             ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, l2cur,
@@ -861,9 +881,13 @@ hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const ui
     const size_t lds = (size_t)P2 * 8 + (size_t)FL_BLOCK * FL_ITEMS * 8 + (size_t)P2 * 12 +
                        (FL_BLOCK / WAVE + 1) * 4;
     const int W = std::max(1, num_cu / 8) * 2;  // workgroups per XCD
-    k_filter_l1<FL_BLOCK, FL_ITEMS><<<8 * W, FL_BLOCK, lds, st>>>(ikey, ival, sb_start, bits, bits2, filter,
-                                                                    g, okey, oval, out_cap, s_fill, count,
-                                                                    err);
+#define BJ_FILTER(K)                                                                                      \
+    k_filter_l1<FL_BLOCK, FL_ITEMS, K><<<8 * W, FL_BLOCK, lds, st>>>(ikey, ival, sb_start, bits, bits2, filter, g, \
+                                                                     okey, oval, out_cap, s_fill, count, err)
+    if (g.k == 3) BJ_FILTER(3);
+    else if (g.k == 4) BJ_FILTER(4);
+    else BJ_FILTER(0);
+#undef BJ_FILTER
     return hipGetLastError();
 }
 
