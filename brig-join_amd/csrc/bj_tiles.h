@@ -304,15 +304,18 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
         const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
         k[it] = (i < n) ? __builtin_nontemporal_load(&skey[i]) : 0u;
     }
+    // every filter word in flight before any is tested; the loads are
+    // unconditional (a lane past n reads the word of key 0): loads under a
+    // branch, with addresses from earlier loads, would each wait for all loads
+    // before them
+    uint64_t fw[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) fw[it] = filter[bloom_block(k[it], g)];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
-        bool ok = false;
-        if (i < n) {
-            const uint64_t m = bloom_mask(k[it], g.k);
-            ok = (filter[bloom_block(k[it], g)] & m) == m;
-        }
-        pass[it] = ok;
+        const uint64_t m = bloom_mask(k[it], g.k);
+        pass[it] = i < n && (fw[it] & m) == m;
     }
     uint32_t rank[ITEMS];
 #pragma unroll
@@ -459,12 +462,15 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         uint32_t kk[PITEMS], qs[PITEMS], qe[PITEMS];
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
-        // prefetch the next round
+        // the next round's keys are issued after this round's value loads, so the
+        // wait for those values does not cover them
+        auto prefetch = [&]() {
 #pragma unroll
-        for (int it = 0; it < PITEMS; it++) {
-            const uint32_t idx = base + ROUND + (uint32_t)it * BLOCK + threadIdx.x;
-            if (idx < pn) nk[it] = pkp[idx];
-        }
+            for (int it = 0; it < PITEMS; it++) {
+                const uint32_t idx = base + ROUND + (uint32_t)it * BLOCK + threadIdx.x;
+                if (idx < pn) nk[it] = pkp[idx];
+            }
+        };
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
             const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
@@ -472,39 +478,42 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
             qs[it] = (idx < pn) ? (uint32_t)L.bst[t] : 0u;
             qe[it] = (idx < pn) ? (uint32_t)L.bst[t + 1] : 0u;
         }
-        // count
-        uint32_t c = 0, hitmask = 0;
+        // count the matches of every key and note its first matching slot q0
+        uint32_t c = 0, hitmask = 0, multi = 0, q0[PITEMS];
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
-            uint32_t m = 0;
-            for (uint32_t q = qs[it]; q < qe[it]; q++) m += (L.hkey[q] == kk[it]);
+            uint32_t m = 0, f = 0;
+            for (uint32_t q = qs[it]; q < qe[it]; q++) {
+                const bool e = L.hkey[q] == kk[it];
+                f = (e && m == 0) ? q : f;
+                m += e;
+            }
+            q0[it] = f;
             c += m;
             hitmask |= (m ? 1u : 0u) << it;
+            multi |= (m > 1 ? 1u : 0u) << it;
         }
         uint32_t total;
         const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, L.wtmp, &total);
-        if (total == 0) continue;  // uniform
+        if (total == 0) {  // uniform
+            prefetch();
+            continue;
+        }
         if (threadIdx.x == 0)
             L.s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
-        // while the reservation is in flight: the first match of every hit key and
-        // the values of both sides, all loads of the round issued together
-        uint32_t q0[PITEMS], pv[PITEMS], bv[PITEMS];
+        // while the reservation is in flight: the values of both sides of every
+        // first match, all loads of the round issued together
+        uint32_t pv[PITEMS], bv[PITEMS];
+        // The loads are unconditional, lanes without a hit read element 0 (one
+        // line per wave): loads under a branch would make the stores below wait
+        // for every load in flight, the next round's keys included.
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
-            q0[it] = 0u;
-            if (hitmask & (1u << it)) {
-                uint32_t q = qs[it];
-                while (L.hkey[q] != kk[it]) q++;
-                q0[it] = q;
-            }
+            const bool hit = (hitmask >> it) & 1u;
+            pv[it] = pvp[hit ? base + (uint32_t)it * BLOCK + threadIdx.x : 0u];
+            bv[it] = bvp[hit ? (uint32_t)L.hidx[q0[it]] : 0u];
         }
-#pragma unroll
-        for (int it = 0; it < PITEMS; it++) {
-            if (hitmask & (1u << it)) {
-                pv[it] = pvp[base + (uint32_t)it * BLOCK + threadIdx.x];
-                bv[it] = bvp[L.hidx[q0[it]]];
-            }
-        }
+        prefetch();
         __syncthreads();
         uint64_t w = L.s_base + pre;
 #pragma unroll
@@ -518,15 +527,17 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
                 }
                 w++;
                 // further matches of the same key (duplicate build keys)
-                for (uint32_t q = q0[it] + 1; q < qe[it]; q++) {
-                    if (L.hkey[q] == kk[it]) {
-                        if (w < out_cap) {
-                            const uint32_t bval = bvp[L.hidx[q]];
-                            o_key[w] = kk[it];
-                            o_rval[w] = build_r ? bval : pval;
-                            o_sval[w] = build_r ? pval : bval;
+                if (multi & (1u << it)) {
+                    for (uint32_t q = q0[it] + 1; q < qe[it]; q++) {
+                        if (L.hkey[q] == kk[it]) {
+                            if (w < out_cap) {
+                                const uint32_t bval = bvp[L.hidx[q]];
+                                o_key[w] = kk[it];
+                                o_rval[w] = build_r ? bval : pval;
+                                o_sval[w] = build_r ? pval : bval;
+                            }
+                            w++;
                         }
-                        w++;
                     }
                 }
             }
