@@ -295,38 +295,36 @@ int run_partition(bj_ctx* c, const Plan& pl, int p1, const uint32_t* key, const 
     // more than half of the CU's LDS: one scatter workgroup per CU
     const size_t min_lds = solo ? (size_t)c->max_lds / 2 + 1024 : 0;
     // k_hist is the BJ_K_HIST_* slot; the three slots after it follow in order
-    auto beg = [&](int slot) { return hipEventRecord(c->ev[EV_KB + slot], c->stream); };
-    auto end = [&](int slot) { return hipEventRecord(c->ev[EV_KE + slot], c->stream); };
+    auto beg = [&](int slot) { return tl_begin(c, slot); };
+    auto end = [&](int slot) { return tl_end(c, slot); };
     hipStream_t st = c->stream;
-    const int nbins = 1 << pl.bits;
     const int b2 = pl.bits - p1;
-    const int P1 = 1 << p1;
     uint64_t* err = P<uint64_t>(c->counters) + CTR_ERR;
     uint32_t* part = P<uint32_t>(c->part);
     BJ_HIP(beg(k_hist));
-    BJ_HIP(launch_hist(key, n_host, n_dev, G, pl.bits, part, st));
+    BJ_HIP(launch_hist(key, n_host, n_dev, G, pl.bits, part, P<uint32_t>(c->totals), st));
     BJ_HIP(end(k_hist));
     BJ_HIP(beg(k_hist + 1));
-    BJ_HIP(launch_reduce(part, G, pl.bits, b2, P<uint32_t>(c->totals), P<uint32_t>(c->l1h), st));
-    BJ_HIP(launch_scan(P<uint32_t>(c->totals), (uint64_t)nbins, start, P<uint64_t>(c->partials), st));
-    const uint32_t* l1h = b2 > 0 ? P<uint32_t>(c->l1h) : part;
-    BJ_HIP(launch_l1_offsets(l1h, G, P1, b2, start, P<uint64_t>(c->l1off), st));
     const bool two_pass = b2 > 0 && !l1_only;
     // enough second-pass workgroups to fill the chip: about 1024 in total
     const int W2 = std::min(std::max(1, std::min(64, 1024 >> p1)), G);
-    if (two_pass) BJ_HIP(launch_l2_offsets(part, G, pl.bits, W2, start, P<uint64_t>(c->l2cur), st));
+    // chunk groups of the column sums: the second pass's groups, else up to 8
+    const int W = two_pass ? W2 : std::min(8, G);
+    BJ_HIP(launch_offsets(part, G, pl.bits, b2, W, P<uint32_t>(c->colsum), P<uint32_t>(c->totals),
+                          P<uint32_t>(c->l1h), start,
+                          P<uint64_t>(c->l1off), st));
     BJ_HIP(end(k_hist + 1));
     uint32_t* k1 = two_pass ? P<uint32_t>(c->tmp_key) : out_key;
     uint32_t* v1 = two_pass ? P<uint32_t>(c->tmp_val) : out_val;
     const uint64_t cap1 = two_pass ? c->tmp_key.bytes / 4 : out_cap;
     BJ_HIP(beg(k_hist + 2));
     BJ_HIP(launch_scatter(key, val, k1, v1, cap1, n_host, n_dev, G, 0, pl.bits, p1, W2,
-                          P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, pl.scatter_wg, min_lds, st));
+                          P<uint64_t>(c->l1off), start, P<uint32_t>(c->colsum), err, pl.scatter_wg, min_lds, st));
     BJ_HIP(end(k_hist + 2));
     BJ_HIP(beg(k_hist + 3));
     if (two_pass)
         BJ_HIP(launch_scatter(k1, v1, out_key, out_val, out_cap, n_host, n_dev, G, 1, pl.bits, p1, W2,
-                              P<uint64_t>(c->l1off), start, P<uint64_t>(c->l2cur), err, pl.scatter_wg, min_lds, st));
+                              P<uint64_t>(c->l1off), start, P<uint32_t>(c->colsum), err, pl.scatter_wg, min_lds, st));
     BJ_HIP(end(k_hist + 3));
     return BJ_OK;
 }
@@ -358,10 +356,10 @@ int workspace_items(const Plan& pl, bool with_r, bool with_out,
         if ((rc = need(&bj_ctx::s_fill, nbins * 4))) return rc;
     }
     if ((rc = need(&bj_ctx::part, gmax * nbins * 4))) return rc;
-    if ((rc = need(&bj_ctx::totals, nbins * 4))) return rc;
     if ((rc = need(&bj_ctx::l1h, n_l1 * 4))) return rc;
     if ((rc = need(&bj_ctx::l1off, n_l1 * 8))) return rc;
-    if ((rc = need(&bj_ctx::l2cur, w2max * nbins * 8))) return rc;
+    if ((rc = need(&bj_ctx::totals, nbins * 4))) return rc;
+    if ((rc = need(&bj_ctx::colsum, std::max<uint64_t>(w2max, 8) * nbins * 4))) return rc;
     if ((rc = need(&bj_ctx::plan_cnt, nbins * 4))) return rc;
     if ((rc = need(&bj_ctx::item_start, (nbins + 1) * 8))) return rc;
     if ((rc = need(&bj_ctx::s_start, (nbins + 1) * 8))) return rc;
@@ -391,12 +389,58 @@ int ensure_workspace(bj_ctx* c, const Plan& pl, uint64_t* total_bytes)
 float ev_ms(bj_ctx* c, int a, int b)
 {
     float ms = 0.f;
+    if (a < 0 || b < 0) return 0.f;
     if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) {
         (void)hipGetLastError();
         return 0.f;
     }
     return ms;
 }
+
+void tl_reset(bj_ctx* c, bool timing)
+{
+    c->timing = timing;
+    for (int i = 0; i < BJ_NUM_KERNELS; i++) c->tl_b[i] = c->tl_e[i] = -1;
+    c->tl_next = EV_KB;
+    c->tl_last = -1;
+    c->tl_open = false;
+}
+
+static hipError_t tl_mark(bj_ctx* c, int& idx)
+{
+    if (c->tl_next >= EV_N) {
+        idx = -1;  // pool exhausted: the slot reads as 0 ms
+        return hipSuccess;
+    }
+    idx = c->tl_next++;
+    c->tl_last = idx;
+    return hipEventRecord(c->ev[idx], c->stream);
+}
+
+// A slot that starts right after another one ends (nothing enqueued between)
+// starts at that slot's end event.
+hipError_t tl_begin(bj_ctx* c, int slot)
+{
+    if (!c->timing) return hipSuccess;
+    if (c->tl_open) {
+        c->tl_b[slot] = c->tl_last;
+        c->tl_open = false;
+        return hipSuccess;
+    }
+    return tl_mark(c, c->tl_b[slot]);
+}
+
+hipError_t tl_end(bj_ctx* c, int slot)
+{
+    if (!c->timing) return hipSuccess;
+    const hipError_t e = tl_mark(c, c->tl_e[slot]);
+    c->tl_open = true;
+    return e;
+}
+
+float tl_ms(bj_ctx* c, int slot) { return ev_ms(c, c->tl_b[slot], c->tl_e[slot]); }
+
+float tl_span_ms(bj_ctx* c, int first, int last) { return ev_ms(c, c->tl_b[first], c->tl_e[last]); }
 
 int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* r_val)
 {
@@ -405,10 +449,10 @@ int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* 
                             P<uint32_t>(c->r_val), c->r_key.bytes / 4, P<uint64_t>(c->r_start), BJ_K_HIST_R)))
         return rc;
     if (pl.bf) {
-        BJ_HIP(hipEventRecord(c->ev[EV_KB + BJ_K_BF_BUILD], c->stream));
+        BJ_HIP(tl_begin(c, BJ_K_BF_BUILD));
         BJ_HIP(launch_bloom_build(P<uint32_t>(c->r_key), P<uint64_t>(c->r_start), pl.bits, pl.geom,
                                   P<uint64_t>(c->filter), c->stream));
-        BJ_HIP(hipEventRecord(c->ev[EV_KE + BJ_K_BF_BUILD], c->stream));
+        BJ_HIP(tl_end(c, BJ_K_BF_BUILD));
     }
     return BJ_OK;
 }
@@ -418,8 +462,8 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
 {
     hipStream_t st = c->stream;
     uint64_t* ctr = P<uint64_t>(c->counters);
-    auto beg = [&](int slot) { return hipEventRecord(c->ev[EV_KB + slot], st); };
-    auto end = [&](int slot) { return hipEventRecord(c->ev[EV_KE + slot], st); };
+    auto beg = [&](int slot) { return tl_begin(c, slot); };
+    auto end = [&](int slot) { return tl_end(c, slot); };
     int rc;
     const uint32_t* s_cnt = nullptr;
     if (pl.bf) {
@@ -447,7 +491,7 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
             BJ_HIP(launch_filter_l1(P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val), P<uint64_t>(c->s_start),
                                     pl.bits, pl.bits - pl.p1_s, R.filter, pl.geom, P<uint32_t>(c->s_key),
                                     P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint32_t>(c->s_fill),
-                                    ctr + CTR_SPRIME, ctr + CTR_ERR, c->num_cu, st));
+                                    ctr + CTR_SPRIME, ctr + CTR_XCD, ctr + CTR_ERR, c->num_cu, st));
             s_cnt = P<uint32_t>(c->s_fill);
             BJ_HIP(end(BJ_K_BF_FILTER));
         }
@@ -567,7 +611,7 @@ int bj_destroy(bj_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->in_rk,  &c->in_rv,    &c->in_sk,   &c->in_sv,    &c->r_key,   &c->r_val,
                       &c->s_key,  &c->s_val,    &c->tmp_key, &c->tmp_val,  &c->sp_key,  &c->sp_val,
-                      &c->filter, &c->part,     &c->totals,  &c->l1h,      &c->l1off,   &c->l2cur,   &c->r_start,
+                      &c->filter, &c->part,     &c->totals,  &c->l1h,      &c->l1off,   &c->colsum,  &c->r_start,
                       &c->s_start, &c->s_fill,  &c->partials, &c->items,  &c->counters, &c->out_key,
                       &c->out_rval, &c->out_sval, &c->dtmp,  &c->plan_cnt, &c->item_start};
     for (DevBuf* b : bufs) release(*b);
@@ -623,8 +667,12 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
     if ((rc = ensure_workspace(c, pl))) return rc;
     hipStream_t st = c->stream;
 
+    // Kernel-slot events only when the caller collects phase times: each event
+    // between two kernels costs the device several microseconds.
+    tl_reset(c, p->collect_phase_times != 0);
     // host inputs: stage them on the device first (timed as BJ_PH_H2D)
-    BJ_HIP(hipEventRecord(c->ev[EV_H2D0], st));
+    const bool h2d_ev = !p->inputs_on_device && p->collect_phase_times;
+    if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D0], st));
     if (!p->inputs_on_device) {
         if ((rc = ensure(c->in_rk, pl.nr * 4)) || (rc = ensure(c->in_rv, pl.nr * 4)) ||
             (rc = ensure(c->in_sk, pl.ns * 4)) || (rc = ensure(c->in_sv, pl.ns * 4)))
@@ -642,21 +690,11 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
         s_key = P<uint32_t>(c->in_sk);
         s_val = P<uint32_t>(c->in_sv);
     }
-    BJ_HIP(hipEventRecord(c->ev[EV_H2D1], st));
+    if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D1], st));
 
     uint64_t* ctr = P<uint64_t>(c->counters);
     BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
-    auto beg = [&](int slot) { return hipEventRecord(c->ev[EV_KB + slot], st); };
-    auto end = [&](int slot) { return hipEventRecord(c->ev[EV_KE + slot], st); };
-    auto skip = [&](int slot) {
-        hipError_t e = beg(slot);
-        return e == hipSuccess ? end(slot) : e;
-    };
-
-    if (!pl.bf) {
-        BJ_HIP(skip(BJ_K_BF_BUILD));
-        BJ_HIP(skip(BJ_K_BF_FILTER));
-    }
+    // slots without kernels (no filter) keep no events and read as 0 ms
     if ((rc = enqueue_r(c, pl, r_key, r_val))) return rc;
     const RSide R{P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start), P<uint64_t>(c->filter)};
     if ((rc = enqueue_s_probe(c, pl, R, s_key, s_val, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
@@ -691,14 +729,14 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
     out->s_pass1_bits = pl.p1_s;
     if (p->collect_phase_times) {
         double* k = out->kernel_ms;
-        for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = ev_ms(c, EV_KB + i, EV_KE + i);
-        out->phase_ms[BJ_PH_H2D] = ev_ms(c, EV_H2D0, EV_H2D1);
+        for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = tl_ms(c, i);
+        out->phase_ms[BJ_PH_H2D] = h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0;
         out->phase_ms[BJ_PH_R_PART] = k[BJ_K_HIST_R] + k[BJ_K_SCAN_R] + k[BJ_K_SCATTER1_R] + k[BJ_K_SCATTER2_R];
         out->phase_ms[BJ_PH_BF_BUILD] = k[BJ_K_BF_BUILD];
         out->phase_ms[BJ_PH_S_FILTER] = k[BJ_K_BF_FILTER];
         out->phase_ms[BJ_PH_S_PART] = k[BJ_K_HIST_S] + k[BJ_K_SCAN_S] + k[BJ_K_SCATTER1_S] + k[BJ_K_SCATTER2_S];
         out->phase_ms[BJ_PH_PROBE] = k[BJ_K_PLAN] + k[BJ_K_PROBE];
-        out->phase_ms[BJ_PH_TOTAL] = ev_ms(c, EV_KB + BJ_K_HIST_R, EV_KE + BJ_K_PROBE);
+        out->phase_ms[BJ_PH_TOTAL] = tl_span_ms(c, BJ_K_HIST_R, BJ_K_PROBE);
     }
     return BJ_OK;
 }
@@ -719,6 +757,7 @@ int bj_partition(bj_ctx* c, const bj_params* p, const uint32_t* key, const uint3
     if ((rc = ensure_workspace(c, pl))) return rc;
     hipStream_t st = c->stream;
     uint64_t* ctr = P<uint64_t>(c->counters);
+    tl_reset(c, false);
     BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
     if ((rc = run_partition(c, pl, pl.p1, key, val, pl.nr, nullptr, pl.G_r, pl.solo_r, out_key, out_val, pl.nr, out_start,
                             BJ_K_HIST_R)))

@@ -40,10 +40,12 @@ uint64_t splitmix64(uint64_t x);
 PermKeys perm_keys(uint64_t seed);
 int ceil_log2(uint64_t n);
 
-enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_N = 8 };
+// CTR_XCD .. CTR_XCD + 7: tile counters of the filter kernel, one per XCD
+enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_XCD = 8, CTR_N = 16 };
 
-// Kernel slot k (BJ_K_*) is bracketed by events EV_KB + k and EV_KE + k.
-enum { EV_H2D0 = 0, EV_H2D1, EV_KB, EV_KE = EV_KB + BJ_NUM_KERNELS, EV_N = EV_KE + BJ_NUM_KERNELS };
+// Events EV_H2D0/1 bracket the host copies; EV_KB .. EV_N - 1 are a pool for
+// the kernel-slot boundaries (tl_begin / tl_end).
+enum { EV_H2D0 = 0, EV_H2D1, EV_KB, EV_N = EV_KB + 2 * BJ_NUM_KERNELS };
 
 // The plan of one join (bj_params -> radix bits, filter geometry, sizes).
 struct Plan {
@@ -85,11 +87,18 @@ struct bj_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev[bj::EV_N] = {};
+    // timeline of the current join: slot k ran from event tl_b[k] to tl_e[k]
+    // (-1: no event); one event serves as the end of a slot and the start of the
+    // next, and nothing is recorded when the join does not collect times
+    bool timing = true;
+    int tl_b[BJ_NUM_KERNELS] = {}, tl_e[BJ_NUM_KERNELS] = {};
+    int tl_next = bj::EV_KB, tl_last = -1;
+    bool tl_open = false;
     // inputs copied from the host (only when inputs_on_device == 0)
     bj::DevBuf in_rk, in_rv, in_sk, in_sv;
     // partitioned relations and scratch
     bj::DevBuf r_key, r_val, s_key, s_val, tmp_key, tmp_val, sp_key, sp_val;
-    bj::DevBuf filter, part, totals, l1h, l1off, l2cur, r_start, s_start, s_fill, partials, items, counters;
+    bj::DevBuf filter, part, totals, l1h, l1off, colsum, r_start, s_start, s_fill, partials, items, counters;
     bj::DevBuf plan_cnt, item_start;
     bj::DevBuf out_key, out_rval, out_sval, dtmp;
     uint64_t* h_counters = nullptr;  // pinned
@@ -112,5 +121,11 @@ int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* 
 int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
                     uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, uint64_t cap);
 float ev_ms(bj_ctx* c, int a, int b);
+// Kernel-slot timeline (see bj_ctx::tl_*).
+void tl_reset(bj_ctx* c, bool timing);
+hipError_t tl_begin(bj_ctx* c, int slot);
+hipError_t tl_end(bj_ctx* c, int slot);
+float tl_ms(bj_ctx* c, int slot);
+float tl_span_ms(bj_ctx* c, int first, int last);
 
 }  // namespace bj

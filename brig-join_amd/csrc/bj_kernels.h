@@ -25,38 +25,39 @@ hipError_t launch_gen_s(uint32_t* key, uint32_t* val, uint64_t n_s, uint64_t n_r
                         uint32_t useed, double zipf_s, uint32_t zseed, double* d_tmp,
                         hipStream_t st);
 
+// also zeroes totals (2^bits entries) for launch_offsets when not null
 hipError_t launch_hist(const uint32_t* key, uint64_t n_host, const uint64_t* n_dev, int G, int bits,
-                       uint32_t* part, hipStream_t st);
+                       uint32_t* part, uint32_t* totals, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t st);
-// totals[b] = sum_g part[g][b]; l1h[g][b1] = sum_b2 part[g][b1 << bits2 | b2] (when bits2 > 0)
-hipError_t launch_reduce(const uint32_t* part, int G, int bits, int bits2, uint32_t* totals,
-                         uint32_t* l1h, hipStream_t st);
-hipError_t launch_l1_offsets(const uint32_t* l1h, int G, int P1, int bits2, const uint64_t* sb_start,
-                             uint64_t* l1off, hipStream_t st);
+// Sub-bucket starts, pass-1 cursors l1off[g][b1] and per-group column sums
+// colsum[w][b] (W groups of chunks) from the per-chunk histograms (bits <= 15);
+// totals (zeroed by launch_hist) and l1h (G << (bits - bits2) entries) are
+// scratch.
+hipError_t launch_offsets(const uint32_t* part, int G, int bits, int bits2, int W, uint32_t* colsum,
+                          uint32_t* totals, uint32_t* l1h, uint64_t* sb_start, uint64_t* l1off, hipStream_t st);
 size_t scan_partials_needed(uint64_t n);
 hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* partial,
                        hipStream_t st);
 int scatter_carry(int nbk);
 size_t scatter_lds_bytes(int nbk);
 // pass 1 (mode 0): p1-bit buckets; pass 2 (mode 1): the remaining bits, W2
-// workgroups per pass-1 bucket with cursors l2cur[w][sb].  The launch reserves at
+// workgroups per pass-1 bucket, cursors from sb_start and colsum.  The launch reserves at
 // least min_lds bytes of LDS per workgroup (a way to cap workgroups per CU).
 hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* okey, uint32_t* oval,
                           uint64_t out_cap, uint64_t n_host, const uint64_t* n_dev, int G, int mode,
                           int bits, int p1, int W2, const uint64_t* l1off, const uint64_t* sb_start,
-                          const uint64_t* l2cur, uint64_t* err, int wg, size_t min_lds, hipStream_t st);
-hipError_t launch_l2_offsets(const uint32_t* part, int G, int bits, int W2, const uint64_t* sb_start,
-                             uint64_t* l2cur, hipStream_t st);
+                          const uint32_t* colsum, uint64_t* err, int wg, size_t min_lds, hipStream_t st);
 hipError_t launch_bloom_build(const uint32_t* rkey, const uint64_t* r_start, int bits,
                               const BloomGeom& g, uint64_t* filter, hipStream_t st);
 hipError_t launch_bloom_prefilter(const uint32_t* skey, const uint32_t* sval, uint64_t n,
                                   const uint64_t* filter, const BloomGeom& g, uint32_t* okey,
                                   uint32_t* oval, uint64_t out_cap, uint64_t* count, uint64_t* err,
                                   int num_cu, hipStream_t st);
+// tile_ctr: 8 zeroed counters, one per XCD (the tile order of each XCD)
 hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const uint64_t* sb_start, int bits,
                             int bits2, const uint64_t* filter, const BloomGeom& g, uint32_t* okey,
                             uint32_t* oval, uint64_t out_cap, uint32_t* s_fill, uint64_t* count,
-                            uint64_t* err, int num_cu, hipStream_t st);
+                            uint64_t* tile_ctr, uint64_t* err, int num_cu, hipStream_t st);
 hipError_t launch_plan(const uint64_t* r_start, const uint64_t* s_start, const uint32_t* s_cnt, int bits,
                        uint32_t tile, uint32_t slice, uint32_t max_slices, uint32_t* counts,
                        uint64_t* item_start, uint64_t* partials, HeavyItem* items, uint64_t item_cap,

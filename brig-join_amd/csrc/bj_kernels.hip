@@ -1,16 +1,16 @@
 // bj_kernels.hip -- gfx950 kernels of the radix hash join, and their launchers.
 //
 // Pipeline (one GPU; all integer, all exact):
-//   R:  k_hist -> k_reduce_hist -> scans -> k_scatter(pass 1) -> k_scatter(pass 2)
+//   R:  k_hist -> k_colsums, k_starts, k_l1_offsets -> k_scatter(pass 1) -> k_scatter(pass 2)
 //   BF: k_bloom_build (one workgroup per filter segment, built in LDS)
 //   S:  bf_stage 2: k_hist + pass 1 of S, then k_filter_l1 (per first-level
 //       bucket, XCD-aware, filter slice L2-resident) places the survivors;
 //       bf_stage 1: k_bloom_prefilter (S -> S') then the two-level partition
-//   J:  k_plan_count/k_plan_fill (vlsplit work items) -> k_probe (persistent;
+//   J:  k_plan (vlsplit work items) -> k_probe (persistent;
 //       LDS chained hash table on the smaller side, stream the other side)
 //
 // Reference counterparts (Chanho-LEE-SNU/BRIG-Join):
-//   k_hist / k_reduce_hist / scans   gpu_run_hist_for_both, gpu_make_simple_prefix,
+//   k_hist / k_colsums / k_starts    gpu_run_hist_for_both, gpu_make_simple_prefix,
 //                                    gpu_run_hist_second_pass
 //                                    (src/kernel_functions.cu:22-90, 461-492, 163-207)
 //   k_scatter                        gpu_make_sorted_table_for_both, gpu_build_table_second_pass
@@ -125,11 +125,15 @@ __global__ void k_zipf_Hn(double* out, uint32_t n, double s)
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_hist(const uint32_t* __restrict__ key, uint64_t n_host,
                                                 const uint64_t* __restrict__ n_dev, int G,
-                                                int bits, uint32_t* __restrict__ part)
+                                                int bits, uint32_t* __restrict__ part,
+                                                uint32_t* __restrict__ totals)
 {
     extern __shared__ uint32_t h[];
     const int nbins = 1 << bits;
     for (int i = threadIdx.x; i < nbins; i += BLOCK) h[i] = 0u;
+    // the bin totals that k_colsums accumulates start at zero
+    if (totals)
+        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < nbins; i += G * BLOCK) totals[i] = 0u;
     __syncthreads();
     const uint64_t n = n_dev ? *n_dev : n_host;
     const uint64_t chunk = (n + (uint64_t)G - 1) / (uint64_t)G;
@@ -141,34 +145,67 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const uint32_t* __restrict__ key
     for (int b = threadIdx.x; b < nbins; b += BLOCK) dst[b] = h[b];
 }
 
-// totals[b] += sum over a chunk of RCH groups of part[g][b]; grid (bins/256, G/RCH).
-// Loads are coalesced across bins; one atomic per (bin, chunk).  totals is zeroed first.
-constexpr int RCH = 32;
-__global__ void k_reduce_totals(const uint32_t* __restrict__ part, int G, int nbins,
-                                uint32_t* __restrict__ totals)
+// Offsets of a partition, three launches.
+// k_colsums: workgroups [0, ncb * W) sum column block cb (1024 bins) of the
+// per-chunk histograms over chunk group w (chunks [w*G/W, (w+1)*G/W)) into
+// colsum[w][b] and, by atomics, into totals[b] (zeroed by k_hist); when the
+// first pass has fewer bits than the partition, the workgroups after those sum
+// each chunk's histogram over the second-level bits: l1h[g][b1].
+// Pass 2 derives its cursors from sb_start and colsum itself.
+__global__ __launch_bounds__(1024) void k_colsums(const uint32_t* __restrict__ part, int G, int bits, int bits2,
+                                                  int W, uint32_t* __restrict__ colsum, uint32_t* __restrict__ totals,
+                                                  uint32_t* __restrict__ l1h)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbins) return;
-    const int g0 = blockIdx.y * RCH, g1 = min(g0 + RCH, G);
-    uint32_t s = 0;
+    constexpr int B = 1024;
+    const int nbins = 1 << bits, P1 = 1 << (bits - bits2);
+    const int ncb = (nbins + B - 1) / B, nA = ncb * W;
+    if ((int)blockIdx.x < nA) {
+        const int cb = blockIdx.x % ncb, w = blockIdx.x / ncb;
+        const int c = cb * B + threadIdx.x;
+        const int g0 = (int)((int64_t)w * G / W), g1 = (int)((int64_t)(w + 1) * G / W);
+        if (c < nbins) {
+            uint32_t sum = 0;
 #pragma unroll 8
-    for (int g = g0; g < g1; g++) s += part[(size_t)g * nbins + b];
-    if (s) atomicAdd(&totals[b], s);
+            for (int g = g0; g < g1; g++) sum += part[(size_t)g * nbins + c];
+            colsum[(size_t)w * nbins + c] = sum;
+            if (sum) atomicAdd(&totals[c], sum);
+        }
+    } else {
+        const int t = (blockIdx.x - nA) * B + threadIdx.x;
+        if (t < P1 * G) {
+            const int g = t / P1, b1 = t % P1;
+            const uint32_t* src = part + (size_t)g * nbins + ((size_t)b1 << bits2);
+            uint32_t sum = 0;
+            for (int b2 = 0; b2 < (1 << bits2); b2++) sum += src[b2];
+            l1h[t] = sum;
+        }
+    }
 }
 
-// l1h[g][b1] = sum_{b2} part[g][(b1 << bits2) + b2]  (row g of the first-level histogram)
-__global__ void k_reduce_l1(const uint32_t* __restrict__ part, int G, int bits, int bits2,
-                            uint32_t* __restrict__ l1h)
+// k_starts: one workgroup scans the bin totals into the sub-bucket starts
+// (sb_start[nbins] = number of tuples); thread t owns a run of at most 32 bins.
+__global__ __launch_bounds__(1024) void k_starts(const uint32_t* __restrict__ totals, int nbins,
+                                                 uint64_t* __restrict__ sb_start)
 {
-    const int P1 = 1 << (bits - bits2);
-    const int nbins = 1 << bits;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P1 * G) return;
-    const int g = t / P1, b1 = t % P1;
-    const uint32_t* src = part + (size_t)g * nbins + ((size_t)b1 << bits2);
-    uint32_t s = 0;
-    for (int b2 = 0; b2 < (1 << bits2); b2++) s += src[b2];
-    l1h[t] = s;
+    constexpr int MAXPER = 32;
+    __shared__ uint64_t wtmp[1024 / WAVE + 1];
+    const int per = (nbins + 1023) / 1024;
+    const int b0 = min((int)threadIdx.x * per, nbins), b1 = min(b0 + per, nbins);
+    uint32_t v[MAXPER];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < MAXPER; i++) {
+        v[i] = (b0 + i < b1) ? totals[b0 + i] : 0u;
+        sum += v[i];
+    }
+    uint64_t tot;
+    uint64_t ex = block_exclusive_scan<1024, uint64_t>(sum, wtmp, &tot);
+#pragma unroll
+    for (int i = 0; i < MAXPER; i++) {
+        if (b0 + i < b1) sb_start[b0 + i] = ex;
+        ex += v[i];
+    }
+    if (threadIdx.x == 0) sb_start[nbins] = tot;
 }
 
 // First-level cursors of the pass-1 scatter: l1off[g][b1] = start of bucket b1
@@ -206,6 +243,7 @@ __global__ __launch_bounds__(1024) void k_l1_offsets(const uint32_t* __restrict_
         }
     }
 }
+
 
 __global__ void k_zero_u32(uint32_t* __restrict__ p, uint64_t n)
 {
@@ -287,7 +325,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t* __rest
 //       g; bucket = h >> (32 - p1); cur[b] = l1off[g * nbk + b].
 //   mode 1 (pass 2, bits - p1 bits): workgroup (b1, w) owns the pass-1 output of
 //       bucket b1 written by pass-1 groups [w*G/W2, (w+1)*G/W2); bucket =
-//       (h >> (32 - bits)) & (2^(bits-p1) - 1); cur[b] = l2cur[w][(b1 << (bits-p1)) + b].
+//       (h >> (32 - bits)) & (2^(bits-p1) - 1); cur[b] = sb_start[sb] +
+//       sum_{v < w} colsum[v][sb] with sb = (b1 << (bits-p1)) + b.
 // Writes are bounds-checked against out_cap and flag err on violation.
 // Occupancy targets: two 1024-thread workgroups per CU for the 4096-tuple tile,
 // one for the 8192-tuple tile (launched one per CU), three 512-thread ones.
@@ -299,7 +338,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sc_min_wa
     uint32_t* __restrict__ okey, uint32_t* __restrict__ oval, uint64_t out_cap,
     uint64_t n_host, const uint64_t* __restrict__ n_dev, int G, int mode, int bits, int p1, int W2,
     int cs_log2, const uint64_t* __restrict__ l1off, const uint64_t* __restrict__ sb_start,
-    const uint64_t* __restrict__ l2cur, uint64_t* __restrict__ err)
+    const uint32_t* __restrict__ colsum, uint64_t* __restrict__ err)
 {
     const int b2 = bits - p1;
     const int nbk = (mode == 0) ? (1 << p1) : (1 << b2);
@@ -321,34 +360,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sc_min_wa
         const int g0 = (int)(((int64_t)w * G) / W2), g1 = (int)(((int64_t)(w + 1) * G) / W2);
         lo = (g0 < G) ? l1off[(size_t)g0 * P1 + b1] : sb_start[(uint64_t)(b1 + 1) << b2];
         hi = (g1 < G) ? l1off[(size_t)g1 * P1 + b1] : sb_start[(uint64_t)(b1 + 1) << b2];
+        // cursor of sub-bucket sb: its start plus what the earlier groups hold of it
         const uint64_t nbins = 1ull << bits;
-        for (int b = threadIdx.x; b < nbk; b += BLOCK)
-            L.cur[b] = l2cur[(size_t)w * nbins + ((uint64_t)b1 << b2) + b];
+        for (int b = threadIdx.x; b < nbk; b += BLOCK) {
+            const uint64_t sb = ((uint64_t)b1 << b2) + b;
+            uint64_t cur = sb_start[sb];
+            for (int v = 0; v < w; v++) cur += colsum[(size_t)v * nbins + sb];
+            L.cur[b] = cur;
+        }
     }
     // ranges are below 2^31 tuples
     const bool bad = scatter_range<BLOCK, ITEMS>(L, ikey + lo, ival + lo, (uint32_t)(hi - lo), nbk, shift,
                                                  bmask, cs_log2, okey, oval, out_cap);
     if (bad) atomicOr((unsigned long long*)err, 1ull);
-}
-
-// Pass-2 cursors.  ps[w][sb] = sum of part[g][sb] over pass-1 groups g in
-// group w (w < W2); then an exclusive scan over w per sub-bucket, offset by
-// sb_start[sb]: l2cur[w][sb] = where pass-2 workgroup (b1, w) writes its first
-// tuple of sub-bucket sb.  Coalesced across sub-buckets.
-__global__ void k_l2_offsets(const uint32_t* __restrict__ part, int G, int nbins, int W2,
-                             const uint64_t* __restrict__ sb_start, uint64_t* __restrict__ l2cur)
-{
-    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sb >= nbins) return;
-    uint64_t acc = sb_start[sb];
-    for (int w = 0; w < W2; w++) {
-        l2cur[(size_t)w * nbins + sb] = acc;
-        const int g0 = (int)(((int64_t)w * G) / W2), g1 = (int)(((int64_t)(w + 1) * G) / W2);
-        uint32_t s = 0;
-#pragma unroll 8
-        for (int g = g0; g < g1; g++) s += part[(size_t)g * nbins + sb];
-        acc += s;
-    }
 }
 
 // ============================================================== Bloom build
@@ -399,20 +423,48 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_prefilter(
 // S has been scattered by its first-pass bucket.  The filter slice of
 // first-pass bucket p is contiguous (segments are the top radix bits), 1/P1 of
 // the filter.  Workgroups are dealt to XCDs round-robin (workgroup i runs on
-// XCD i % 8), so XCD x walks buckets x, x+8, ... with all its workgroups on the
-// same bucket at the same time: that bucket's filter slice stays in the XCD's
-// L2 while its S tuples stream past (loaded non-temporally).  Survivors are
-// counting-sorted by sub-bucket in LDS per tile, reserved with one atomic per
-// (tile, sub-bucket) and written, compacted, into their sub-bucket's region of
-// the S layout (sub-bucket sb gets positions sb_start[sb] + [0, s_fill[sb])),
-// so no second pass over S is needed.  Values are read only for survivors.
+// XCD i % 8), so XCD x takes buckets x, x+8, ... as one sequence of tiles,
+// handed out in order by the XCD's own counter (tile_ctr[x]): all its
+// workgroups work on the same bucket, whose filter slice stays in the XCD's L2
+// while its S tuples stream past (loaded non-temporally), and a workgroup that
+// finishes early takes the next tile instead of running ahead into the next
+// bucket.  Survivors are counting-sorted by sub-bucket in LDS per tile, reserved
+// with one atomic per (tile, sub-bucket) and written, compacted, into their
+// sub-bucket's region of the S layout (sub-bucket sb gets positions
+// sb_start[sb] + [0, s_fill[sb])), so no second pass over S is needed.  Values
+// are read only for survivors.
+//
+// Tile walk of one XCD: bucket p owns ceil(size / T) consecutive tile numbers
+// starting at tb; a workgroup's tile numbers only grow, so it walks forward.
+struct FilterWalk {
+    int p;          // current bucket (>= P1: done)
+    uint64_t tb;    // first tile number of bucket p
+    uint64_t lo, hi;
+};
+
+__device__ __forceinline__ void filter_walk_to(FilterWalk& w, uint64_t t, const uint64_t* __restrict__ sb_start,
+                                               int P1, int bits2, uint64_t T)
+{
+    while (w.p < P1) {
+        const uint64_t nt = (w.hi - w.lo + T - 1) / T;
+        if (t < w.tb + nt) return;
+        w.tb += nt;
+        w.p += 8;
+        if (w.p < P1) {
+            w.lo = sb_start[(uint64_t)w.p << bits2];
+            w.hi = sb_start[(uint64_t)(w.p + 1) << bits2];
+        }
+    }
+}
+
 // K: the filter's bits per key when fixed at compile time (3 or 4), 0 = g.k.
 template <int BLOCK, int ITEMS, int K>
 __global__ __launch_bounds__(BLOCK) void k_filter_l1(
     const uint32_t* __restrict__ ikey, const uint32_t* __restrict__ ival,
     const uint64_t* __restrict__ sb_start, int bits, int bits2, const uint64_t* __restrict__ filter,
     BloomGeom g, uint32_t* __restrict__ okey, uint32_t* __restrict__ oval, uint64_t out_cap,
-    uint32_t* __restrict__ s_fill, uint64_t* __restrict__ count, uint64_t* __restrict__ err)
+    uint32_t* __restrict__ s_fill, uint64_t* __restrict__ count, uint64_t* __restrict__ tile_ctr,
+    uint64_t* __restrict__ err)
 {
     constexpr int T = BLOCK * ITEMS;
     constexpr int NXCD = 8;
@@ -427,129 +479,162 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
     uint32_t* off = cnt0 + 2 * P2;                // [P2]
     uint32_t* wtmp = off + P2;                    // [BLOCK/64 + 1]
     __shared__ uint32_t s_total;
+    __shared__ uint64_t s_tile;
     const bool small = P2 <= 8 * WAVE;
 
     const int xcd = blockIdx.x % NXCD;
-    const int W = gridDim.x / NXCD;
-    const int j = blockIdx.x / NXCD;
-    if (j >= W) return;  // grid is a multiple of NXCD; uniform per workgroup
+    unsigned long long* xctr = (unsigned long long*)tile_ctr + xcd;
     bool bad = false;
     uint32_t surv_total = 0;
     for (int b = threadIdx.x; b < 2 * P2; b += BLOCK) cnt0[b] = 0u;
+    // tile numbers are taken one tile ahead: the current one and the next are
+    // known, the one after that is requested while the current tile runs
+    __shared__ uint64_t s_tile2;
+    if (threadIdx.x == 0) {
+        s_tile = atomicAdd(xctr, 1ull);
+        s_tile2 = atomicAdd(xctr, 1ull);
+    }
     __syncthreads();
+    uint64_t t_next = s_tile2;
     uint32_t par = 0;
 
-    // Per tile three barriers: test + rank | offsets and reservations (one wave;
-    // it also clears the other tile's counts) | staging | out (runs into the
-    // next tile's test).
-    for (int p = xcd; p < P1; p += NXCD) {
-        const uint64_t lo = sb_start[(uint64_t)p << bits2];
-        const uint64_t hi = sb_start[(uint64_t)(p + 1) << bits2];
-        const uint64_t chunk = (hi - lo + W - 1) / W;
-        const uint64_t clo = min(lo + (uint64_t)j * chunk, hi);
-        const uint64_t chi = min(clo + chunk, hi);
-        // keys are streamed once (non-temporal, so the filter slice keeps the L2);
-        // the next tile's keys load while the current tile is written out
-        uint32_t nk[ITEMS];
+    FilterWalk wk;
+    wk.p = xcd;
+    wk.tb = 0;
+    wk.lo = wk.hi = 0;
+    if (wk.p < P1) {
+        wk.lo = sb_start[(uint64_t)wk.p << bits2];
+        wk.hi = sb_start[(uint64_t)(wk.p + 1) << bits2];
+    }
+    filter_walk_to(wk, s_tile, sb_start, P1, bits2, T);
+    // the current tile [tile, chi) of bucket p; keys are streamed once
+    // (non-temporal, so the filter slice keeps the L2) and the next tile's keys
+    // load while the current tile is written out
+    int p = wk.p;
+    uint64_t tile = wk.lo + (s_tile - wk.tb) * (uint64_t)T;
+    uint64_t chi = min(tile + (uint64_t)T, wk.hi);
+    uint32_t nk[ITEMS];
+    if (p < P1) {
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
-            const uint64_t i = clo + (uint64_t)it * BLOCK + threadIdx.x;
+            const uint64_t i = tile + (uint64_t)it * BLOCK + threadIdx.x;
             nk[it] = (i < chi) ? __builtin_nontemporal_load(&ikey[i]) : 0u;
         }
-        for (uint64_t tile = clo; tile < chi; tile += T, par ^= 1u) {
-            const int tn = (int)min((uint64_t)T, chi - tile);
-            uint32_t* cnt = cnt0 + (par ? P2 : 0);
-            uint32_t* cnext = cnt0 + (par ? 0 : P2);
-            uint32_t k[ITEMS], bk[ITEMS], rk[ITEMS];
-            bool pass[ITEMS];
+    }
+
+    // Per tile three barriers: test + rank | offsets and reservations (one wave;
+    // it also clears the other tile's counts and publishes the next tile
+    // number) | staging | out (runs into the next tile's test).
+    while (p < P1) {
+        const int tn = (int)(chi - tile);
+        uint32_t* cnt = cnt0 + (par ? P2 : 0);
+        uint32_t* cnext = cnt0 + (par ? 0 : P2);
+        // the tile number after the next one, in flight during this tile
+        unsigned long long t_after = 0ull;
+        if (threadIdx.x == 0) t_after = atomicAdd(xctr, 1ull);
+        uint32_t k[ITEMS], bk[ITEMS], rk[ITEMS];
+        bool pass[ITEMS];
 #pragma unroll
-            for (int it = 0; it < ITEMS; it++) {
-                const int s = it * BLOCK + threadIdx.x;
-                k[it] = (s < tn) ? nk[it] : 0u;
+        for (int it = 0; it < ITEMS; it++) {
+            const int s = it * BLOCK + threadIdx.x;
+            k[it] = (s < tn) ? nk[it] : 0u;
+        }
+        // all filter words of the tile in flight before any is tested.  The
+        // loads are unconditional (a lane past the tile reads the word of key
+        // 0): loads under a branch, with addresses from earlier loads, would
+        // each wait for all loads before them.
+        uint64_t fw[ITEMS];
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) fw[it] = filter[bloom_block(k[it], g)];
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const int s = it * BLOCK + threadIdx.x;
+            const uint64_t m = K ? bloom_mask_k<(K ? K : 1)>(k[it]) : bloom_mask(k[it], g.k);
+            const bool ok = s < tn && (fw[it] & m) == m;
+            pass[it] = ok;
+            if (ok) {
+                bk[it] = (radix_hash(k[it]) >> shift) & bmask;
+                rk[it] = atomicAdd(&cnt[bk[it]], 1u);
             }
-            // all filter words of the tile in flight before any is tested.  The
-            // loads are unconditional (a lane past the tile reads the word of key
-            // 0): loads under a branch, with addresses from earlier loads, would
-            // each wait for all loads before them.
-            uint64_t fw[ITEMS];
-#pragma unroll
-            for (int it = 0; it < ITEMS; it++) fw[it] = filter[bloom_block(k[it], g)];
-#pragma unroll
-            for (int it = 0; it < ITEMS; it++) {
-                const int s = it * BLOCK + threadIdx.x;
-                const uint64_t m = K ? bloom_mask_k<(K ? K : 1)>(k[it]) : bloom_mask(k[it], g.k);
-                const bool ok = s < tn && (fw[it] & m) == m;
-                pass[it] = ok;
-                if (ok) {
-                    bk[it] = (radix_hash(k[it]) >> shift) & bmask;
-                    rk[it] = atomicAdd(&cnt[bk[it]], 1u);
-                }
-            }
-            __syncthreads();
-            if (small) {
-                if (threadIdx.x < WAVE) {
-                    const int per = (P2 + WAVE - 1) / WAVE;
-                    const int b0 = min((int)threadIdx.x * per, P2), b1 = min(b0 + per, P2);
-                    uint32_t sum = 0;
-                    for (int b = b0; b < b1; b++) sum += cnt[b];
-                    const uint32_t inc = wave_inclusive_scan<uint32_t>(sum);
-                    uint32_t ex = inc - sum;
-                    for (int b = b0; b < b1; b++) {
-                        const uint32_t c = cnt[b];
-                        off[b] = ex;
-                        ex += c;
-                        const uint64_t sb = ((uint64_t)p << bits2) + b;
-                        dst0[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
-                        cnext[b] = 0u;
-                    }
-                    if (threadIdx.x == WAVE - 1) s_total = inc;
-                }
-            } else {
-                const uint32_t tot = lds_exclusive_scan<BLOCK>(cnt, off, P2, wtmp);
-                if (threadIdx.x == 0) s_total = tot;
-                for (int b = threadIdx.x; b < P2; b += BLOCK) {
-                    const uint64_t sb = ((uint64_t)p << bits2) + b;
+        }
+        __syncthreads();
+        if (small) {
+            if (threadIdx.x < WAVE) {
+                const int per = (P2 + WAVE - 1) / WAVE;
+                const int b0 = min((int)threadIdx.x * per, P2), b1 = min(b0 + per, P2);
+                uint32_t sum = 0;
+                for (int b = b0; b < b1; b++) sum += cnt[b];
+                const uint32_t inc = wave_inclusive_scan<uint32_t>(sum);
+                uint32_t ex = inc - sum;
+                for (int b = b0; b < b1; b++) {
                     const uint32_t c = cnt[b];
+                    off[b] = ex;
+                    ex += c;
+                    const uint64_t sb = ((uint64_t)p << bits2) + b;
                     dst0[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
                     cnext[b] = 0u;
                 }
+                if (threadIdx.x == WAVE - 1) s_total = inc;
             }
-            // survivors' values: loads first, then the LDS stores
-            uint32_t vv[ITEMS];
+        } else {
+            const uint32_t tot = lds_exclusive_scan<BLOCK>(cnt, off, P2, wtmp);
+            if (threadIdx.x == 0) s_total = tot;
+            for (int b = threadIdx.x; b < P2; b += BLOCK) {
+                const uint64_t sb = ((uint64_t)p << bits2) + b;
+                const uint32_t c = cnt[b];
+                dst0[b] = sb_start[sb] + (c ? atomicAdd(&s_fill[sb], c) : 0u);
+                cnext[b] = 0u;
+            }
+        }
+        // survivors' values: loads first, then the LDS stores
+        uint32_t vv[ITEMS];
 #pragma unroll
-            for (int it = 0; it < ITEMS; it++)
-                if (pass[it]) vv[it] = __builtin_nontemporal_load(&ival[tile + it * BLOCK + threadIdx.x]);
-            // the next tile's keys, issued last so that no wait of this tile covers
-            // them (clamped rather than skipped past the end, as above)
+        for (int it = 0; it < ITEMS; it++)
+            if (pass[it]) vv[it] = __builtin_nontemporal_load(&ival[tile + it * BLOCK + threadIdx.x]);
+        __syncthreads();
+        // the next tile: its keys are issued after this tile's value loads, so no
+        // wait of this tile covers them (clamped rather than skipped past the end,
+        // as above)
+        filter_walk_to(wk, t_next, sb_start, P1, bits2, T);
+        const int p_next = wk.p;
+        const uint64_t tile_next = wk.lo + (t_next - wk.tb) * (uint64_t)T;
+        const uint64_t chi_next = min(tile_next + (uint64_t)T, wk.hi);
+        if (p_next < P1) {
 #pragma unroll
             for (int it = 0; it < ITEMS; it++) {
-                const uint64_t i = min(tile + T + (uint64_t)it * BLOCK + threadIdx.x, chi - 1);
+                const uint64_t i = min(tile_next + (uint64_t)it * BLOCK + threadIdx.x, chi_next - 1);
                 nk[it] = __builtin_nontemporal_load(&ikey[i]);
             }
-            __syncthreads();
-#pragma unroll
-            for (int it = 0; it < ITEMS; it++) {
-                if (pass[it]) {
-                    const uint32_t q = off[bk[it]] + rk[it];
-                    skey[q] = k[it];
-                    sval[q] = vv[it];
-                }
-            }
-            __syncthreads();
-            const uint32_t ts = s_total;
-            for (uint32_t q = threadIdx.x; q < ts; q += BLOCK) {
-                const uint32_t kk = skey[q];
-                const uint32_t b = (radix_hash(kk) >> shift) & bmask;
-                const uint64_t d = dst0[b] + (q - off[b]);
-                if (d < out_cap) {
-                    okey[d] = kk;
-                    oval[d] = sval[q];
-                } else {
-                    bad = true;
-                }
-            }
-            surv_total += ts;
         }
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            if (pass[it]) {
+                const uint32_t q = off[bk[it]] + rk[it];
+                skey[q] = k[it];
+                sval[q] = vv[it];
+            }
+        }
+        // publish the tile number after the next (thread 0 waits for it only here)
+        if (threadIdx.x == 0) s_tile = t_after;
+        __syncthreads();
+        const uint32_t ts = s_total;
+        for (uint32_t q = threadIdx.x; q < ts; q += BLOCK) {
+            const uint32_t kk = skey[q];
+            const uint32_t b = (radix_hash(kk) >> shift) & bmask;
+            const uint64_t d = dst0[b] + (q - off[b]);
+            if (d < out_cap) {
+                okey[d] = kk;
+                oval[d] = sval[q];
+            } else {
+                bad = true;
+            }
+        }
+        surv_total += ts;
+        p = p_next;
+        tile = tile_next;
+        chi = chi_next;
+        t_next = s_tile;  // read before thread 0 rewrites it after the next tile's third barrier
+        par ^= 1u;
     }
     if (threadIdx.x == 0 && surv_total)
         atomicAdd((unsigned long long*)count, (unsigned long long)surv_total);
@@ -573,43 +658,51 @@ __device__ __forceinline__ void sb_sizes(const uint64_t* r_start, const uint64_t
     ns = s_cnt ? (uint64_t)s_cnt[sb] : (s_start[sb + 1] - s0);
 }
 
-__global__ void k_plan_count(const uint64_t* __restrict__ r_start, const uint64_t* __restrict__ s_start,
-                             const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile,
-                             uint32_t slice, uint32_t max_slices, uint32_t* __restrict__ n_items)
+// One workgroup: thread t owns a run of at most 32 sub-buckets; counts, one
+// block scan into item_start (item_start[nbins] = total), then the items.
+__global__ __launch_bounds__(1024) void k_plan(const uint64_t* __restrict__ r_start,
+                                               const uint64_t* __restrict__ s_start,
+                                               const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile,
+                                               uint32_t slice, uint32_t max_slices,
+                                               uint64_t* __restrict__ item_start, uint64_t item_cap,
+                                               HeavyItem* __restrict__ items, uint64_t* __restrict__ err)
 {
-    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sb >= nbins) return;
-    uint64_t r0, nr, s0, ns;
-    sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
-    uint32_t c, sl;
-    n_items[sb] = plan_items(nr, ns, tile, slice, max_slices, c, sl);
-}
-
-__global__ void k_plan_fill(const uint64_t* __restrict__ r_start, const uint64_t* __restrict__ s_start,
-                            const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile, uint32_t slice,
-                            uint32_t max_slices, const uint64_t* __restrict__ item_start,
-                            uint64_t item_cap, HeavyItem* __restrict__ items, uint64_t* __restrict__ err)
-{
-    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sb >= nbins) return;
-    uint64_t r0, nr, s0, ns;
-    sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
-    uint32_t chunks, slices;
-    const uint32_t n = plan_items(nr, ns, tile, slice, max_slices, chunks, slices);
-    const uint64_t base = item_start[sb];
-    if (base + n > item_cap) {
-        atomicOr((unsigned long long*)err, 4ull);
-        return;
+    __shared__ uint64_t wtmp[1024 / WAVE + 1];
+    const int per = (nbins + 1023) / 1024;
+    const int b0 = min((int)threadIdx.x * per, nbins), b1 = min(b0 + per, nbins);
+    uint64_t sum = 0;
+    for (int sb = b0; sb < b1; sb++) {
+        uint64_t r0, nr, s0, ns;
+        sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
+        uint32_t c, sl;
+        sum += plan_items(nr, ns, tile, slice, max_slices, c, sl);
     }
-    for (uint32_t c = 0; c < chunks; c++)
-        for (uint32_t s = 0; s < slices; s++) {
-            HeavyItem it;
-            it.sb = (uint32_t)sb;
-            it.chunk = c;
-            it.slice = s;
-            it.nslices = slices;
-            items[base + (uint64_t)c * slices + s] = it;
+    uint64_t total;
+    uint64_t base = block_exclusive_scan<1024, uint64_t>(sum, wtmp, &total);
+    if (threadIdx.x == 0) item_start[nbins] = total;
+    bool over = false;
+    for (int sb = b0; sb < b1; sb++) {
+        uint64_t r0, nr, s0, ns;
+        sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
+        uint32_t chunks, slices;
+        const uint32_t n = plan_items(nr, ns, tile, slice, max_slices, chunks, slices);
+        item_start[sb] = base;
+        if (base + n > item_cap) {
+            over = true;
+        } else {
+            for (uint32_t c = 0; c < chunks; c++)
+                for (uint32_t q = 0; q < slices; q++) {
+                    HeavyItem it;
+                    it.sb = (uint32_t)sb;
+                    it.chunk = c;
+                    it.slice = q;
+                    it.nslices = slices;
+                    items[base + (uint64_t)c * slices + q] = it;
+                }
         }
+        base += n;
+    }
+    if (over) atomicOr((unsigned long long*)err, 4ull);
 }
 
 // ============================================================== build + probe
@@ -746,10 +839,10 @@ hipError_t kernels_init(int max_lds_bytes)
 }
 
 hipError_t launch_hist(const uint32_t* key, uint64_t n_host, const uint64_t* n_dev, int G, int bits,
-                       uint32_t* part, hipStream_t st)
+                       uint32_t* part, uint32_t* totals, hipStream_t st)
 {
     const size_t lds = sizeof(uint32_t) << bits;
-    k_hist<HIST_BLOCK><<<G, HIST_BLOCK, lds, st>>>(key, n_host, n_dev, G, bits, part);
+    k_hist<HIST_BLOCK><<<G, HIST_BLOCK, lds, st>>>(key, n_host, n_dev, G, bits, part, totals);
     return hipGetLastError();
 }
 
@@ -761,25 +854,15 @@ hipError_t launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_reduce(const uint32_t* part, int G, int bits, int bits2, uint32_t* totals,
-                         uint32_t* l1h, hipStream_t st)
+hipError_t launch_offsets(const uint32_t* part, int G, int bits, int bits2, int W, uint32_t* colsum,
+                          uint32_t* totals, uint32_t* l1h, uint64_t* sb_start, uint64_t* l1off, hipStream_t st)
 {
-    const int nbins = 1 << bits;
-    hipError_t e = launch_zero_u32(totals, (uint64_t)nbins, st);
-    if (e != hipSuccess) return e;
-    dim3 grid(ceil_div_i(nbins, 256), ceil_div_i(G, RCH));
-    k_reduce_totals<<<grid, 256, 0, st>>>(part, G, nbins, totals);
-    if (bits2 > 0) {
-        const int n1 = (1 << (bits - bits2)) * G;
-        k_reduce_l1<<<ceil_div_i(n1, 256), 256, 0, st>>>(part, G, bits, bits2, l1h);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_l1_offsets(const uint32_t* l1h, int G, int P1, int bits2, const uint64_t* sb_start,
-                             uint64_t* l1off, hipStream_t st)
-{
-    k_l1_offsets<<<ceil_div_i(P1, 64), 1024, 0, st>>>(l1h, G, P1, bits2, sb_start, l1off);
+    if (bits > 15) return hipErrorInvalidValue;
+    const int nbins = 1 << bits, P1 = 1 << (bits - bits2);
+    const int grid = ceil_div_i(nbins, 1024) * W + (bits2 ? ceil_div_i((uint64_t)P1 * G, 1024) : 0);
+    k_colsums<<<grid, 1024, 0, st>>>(part, G, bits, bits2, W, colsum, totals, l1h);
+    k_starts<<<1, 1024, 0, st>>>(totals, nbins, sb_start);
+    k_l1_offsets<<<ceil_div_i(P1, 64), 1024, 0, st>>>(bits2 ? l1h : part, G, P1, bits2, sb_start, l1off);
     return hipGetLastError();
 }
 
@@ -816,18 +899,10 @@ static size_t scatter_lds_tile(int nbk, int tile)
 
 size_t scatter_lds_bytes(int nbk) { return scatter_lds_tile(nbk, SC_BLOCK * SC_ITEMS); }
 
-hipError_t launch_l2_offsets(const uint32_t* part, int G, int bits, int W2, const uint64_t* sb_start,
-                             uint64_t* l2cur, hipStream_t st)
-{
-    const int nbins = 1 << bits;
-    k_l2_offsets<<<ceil_div_i(nbins, 256), 256, 0, st>>>(part, G, nbins, W2, sb_start, l2cur);
-    return hipGetLastError();
-}
-
 hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* okey, uint32_t* oval,
                           uint64_t out_cap, uint64_t n_host, const uint64_t* n_dev, int G, int mode,
                           int bits, int p1, int W2, const uint64_t* l1off, const uint64_t* sb_start,
-                          const uint64_t* l2cur, uint64_t* err, int wg, size_t min_lds, hipStream_t st)
+                          const uint32_t* colsum, uint64_t* err, int wg, size_t min_lds, hipStream_t st)
 {
     const int nbk = (mode == 0) ? (1 << p1) : (1 << (bits - p1));
     const int grid = (mode == 0) ? G : (1 << p1) * W2;
@@ -837,16 +912,16 @@ hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* 
     if (wg != 512 && min_lds > 0 && lds_big <= g_max_lds) {
         // one workgroup per CU: twice the tile, twice the loads in flight
         k_scatter<SC_BLOCK, SC_BIG_ITEMS><<<grid, SC_BLOCK, lds_big, st>>>(
-            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, l2cur,
+            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, colsum,
             err);
     } else if (wg == 512) {
         // same tile (4096 tuples), half the threads: more workgroups per CU
         k_scatter<SC_BLOCK / 2, SC_ITEMS * 2><<<grid, SC_BLOCK / 2, lds, st>>>(
-            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, l2cur,
+            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, colsum,
             err);
     } else {
         k_scatter<SC_BLOCK, SC_ITEMS><<<grid, SC_BLOCK, lds, st>>>(
-            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, l2cur,
+            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, colsum,
             err);
     }
     return hipGetLastError();
@@ -875,7 +950,7 @@ hipError_t launch_bloom_prefilter(const uint32_t* skey, const uint32_t* sval, ui
 hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const uint64_t* sb_start, int bits,
                             int bits2, const uint64_t* filter, const BloomGeom& g, uint32_t* okey,
                             uint32_t* oval, uint64_t out_cap, uint32_t* s_fill, uint64_t* count,
-                            uint64_t* err, int num_cu, hipStream_t st)
+                            uint64_t* tile_ctr, uint64_t* err, int num_cu, hipStream_t st)
 {
     const int P2 = 1 << bits2;
     const size_t lds = (size_t)P2 * 8 + (size_t)FL_BLOCK * FL_ITEMS * 8 + (size_t)P2 * 12 +
@@ -883,7 +958,7 @@ hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const ui
     const int W = std::max(1, num_cu / 8) * 2;  // workgroups per XCD
 #define BJ_FILTER(K)                                                                                      \
     k_filter_l1<FL_BLOCK, FL_ITEMS, K><<<8 * W, FL_BLOCK, lds, st>>>(ikey, ival, sb_start, bits, bits2, filter, g, \
-                                                                     okey, oval, out_cap, s_fill, count, err)
+                                                                     okey, oval, out_cap, s_fill, count, tile_ctr, err)
     if (g.k == 3) BJ_FILTER(3);
     else if (g.k == 4) BJ_FILTER(4);
     else BJ_FILTER(0);
@@ -896,13 +971,11 @@ hipError_t launch_plan(const uint64_t* r_start, const uint64_t* s_start, const u
                        uint64_t* item_start, uint64_t* partials, HeavyItem* items, uint64_t item_cap,
                        uint64_t* err, hipStream_t st)
 {
-    const int nbins = 1 << bits;
-    k_plan_count<<<ceil_div_i(nbins, 256), 256, 0, st>>>(r_start, s_start, s_cnt, nbins, tile, slice,
-                                                         max_slices, counts);
-    hipError_t e = launch_scan(counts, (uint64_t)nbins, item_start, partials, st);
-    if (e != hipSuccess) return e;
-    k_plan_fill<<<ceil_div_i(nbins, 256), 256, 0, st>>>(r_start, s_start, s_cnt, nbins, tile, slice,
-                                                        max_slices, item_start, item_cap, items, err);
+    (void)counts;
+    (void)partials;
+    if (bits > 15) return hipErrorInvalidValue;
+    k_plan<<<1, 1024, 0, st>>>(r_start, s_start, s_cnt, 1 << bits, tile, slice, max_slices, item_start, item_cap,
+                               items, err);
     return hipGetLastError();
 }
 

@@ -199,6 +199,7 @@ int launch(bj_server* s, Slot& sl, Request* q)
     uint32_t* osv = (uint32_t*)(base + q->out_off + 2 * col);
     hipStream_t st = c->stream;
     BJ_HIP(hipEventRecord(sl.t0, st));
+    tl_reset(c, false);  // only the request's span is timed
     BJ_HIP(launch_zero_u64(P<uint64_t>(c->counters), CTR_N, st));
     int rc = enqueue_s_probe(c, q->pl, rs, q->sk, q->sv, ok, orv, osv, q->cap);
     if (rc) return rc;
@@ -450,6 +451,7 @@ int bj_server_add_relation(bj_server* s, const uint32_t* r_key, const uint32_t* 
         delete R;
         return rc;
     }
+    tl_reset(c, false);
     e = launch_zero_u64(P<uint64_t>(c->counters), CTR_N, c->stream);
     if (e != hipSuccess) {
         delete R;
