@@ -302,7 +302,7 @@ int run_partition(bj_ctx* c, const Plan& pl, int p1, const uint32_t* key, const 
     uint64_t* err = P<uint64_t>(c->counters) + CTR_ERR;
     uint32_t* part = P<uint32_t>(c->part);
     BJ_HIP(beg(k_hist));
-    BJ_HIP(launch_hist(key, n_host, n_dev, G, pl.bits, part, P<uint32_t>(c->totals), st));
+    BJ_HIP(launch_hist(key, n_host, n_dev, G, pl.bits, part, P<uint32_t>(c->totals), b2, P<uint32_t>(c->l1h), st));
     BJ_HIP(end(k_hist));
     BJ_HIP(beg(k_hist + 1));
     const bool two_pass = b2 > 0 && !l1_only;

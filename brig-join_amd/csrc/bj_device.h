@@ -122,6 +122,15 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// sum over the wave, in every lane
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x)
+{
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) x += __shfl_xor(x, o, WAVE);
+    return x;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_scan(T x)
 {

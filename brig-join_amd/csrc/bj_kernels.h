@@ -25,14 +25,16 @@ hipError_t launch_gen_s(uint32_t* key, uint32_t* val, uint64_t n_s, uint64_t n_r
                         uint32_t useed, double zipf_s, uint32_t zseed, double* d_tmp,
                         hipStream_t st);
 
-// also zeroes totals (2^bits entries) for launch_offsets when not null
+// Also zeroes totals (2^bits entries) for launch_offsets when not null, and
+// with bits2 > 0 writes each chunk's first-level counts l1h[g][b1] (the sums
+// over the low bits2 bits).
 hipError_t launch_hist(const uint32_t* key, uint64_t n_host, const uint64_t* n_dev, int G, int bits,
-                       uint32_t* part, uint32_t* totals, hipStream_t st);
+                       uint32_t* part, uint32_t* totals, int bits2, uint32_t* l1h, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t st);
 // Sub-bucket starts, pass-1 cursors l1off[g][b1] and per-group column sums
-// colsum[w][b] (W groups of chunks) from the per-chunk histograms (bits <= 15);
-// totals (zeroed by launch_hist) and l1h (G << (bits - bits2) entries) are
-// scratch.
+// colsum[w][b] (W groups of chunks) from the per-chunk histograms and, when
+// bits2 > 0, the first-level counts l1h (both from launch_hist; bits <= 15);
+// totals (zeroed by launch_hist) is scratch.
 hipError_t launch_offsets(const uint32_t* part, int G, int bits, int bits2, int W, uint32_t* colsum,
                           uint32_t* totals, uint32_t* l1h, uint64_t* sb_start, uint64_t* l1off, hipStream_t st);
 size_t scan_partials_needed(uint64_t n);

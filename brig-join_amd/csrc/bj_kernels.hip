@@ -6,7 +6,7 @@
 //   S:  bf_stage 2: k_hist + pass 1 of S, then k_filter_l1 (per first-level
 //       bucket, XCD-aware, filter slice L2-resident) places the survivors;
 //       bf_stage 1: k_bloom_prefilter (S -> S') then the two-level partition
-//   J:  k_plan (vlsplit work items) -> k_probe (persistent;
+//   J:  k_plan_count, k_plan_fill (vlsplit work items) -> k_probe (persistent;
 //       LDS chained hash table on the smaller side, stream the other side)
 //
 // Reference counterparts (Chanho-LEE-SNU/BRIG-Join):
@@ -126,7 +126,8 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_hist(const uint32_t* __restrict__ key, uint64_t n_host,
                                                 const uint64_t* __restrict__ n_dev, int G,
                                                 int bits, uint32_t* __restrict__ part,
-                                                uint32_t* __restrict__ totals)
+                                                uint32_t* __restrict__ totals, int bits2,
+                                                uint32_t* __restrict__ l1h)
 {
     extern __shared__ uint32_t h[];
     const int nbins = 1 << bits;
@@ -143,23 +144,34 @@ __global__ __launch_bounds__(BLOCK) void k_hist(const uint32_t* __restrict__ key
     __syncthreads();
     uint32_t* dst = part + (size_t)blockIdx.x * nbins;
     for (int b = threadIdx.x; b < nbins; b += BLOCK) dst[b] = h[b];
+    // this chunk's first-level counts (bits2 > 0): thread b1 sums its 2^bits2
+    // bins from LDS, starting at a rotated offset so that threads hit different
+    // banks at each step
+    if (bits2 > 0) {
+        const int P1 = nbins >> bits2, seg = 1 << bits2;
+        for (int b1 = threadIdx.x; b1 < P1; b1 += BLOCK) {
+            const uint32_t* hb = h + ((size_t)b1 << bits2);
+            uint32_t sum = 0;
+            for (int j = 0; j < seg; j++) sum += hb[(j + b1) & (seg - 1)];
+            l1h[(size_t)blockIdx.x * P1 + b1] = sum;
+        }
+    }
 }
 
 // Offsets of a partition, three launches.
 // k_colsums: workgroups [0, ncb * W) sum column block cb (1024 bins) of the
 // per-chunk histograms over chunk group w (chunks [w*G/W, (w+1)*G/W)) into
-// colsum[w][b] and, by atomics, into totals[b] (zeroed by k_hist); when the
-// first pass has fewer bits than the partition, the workgroups after those sum
-// each chunk's histogram over the second-level bits: l1h[g][b1].
+// colsum[w][b] and, by atomics, into totals[b] (zeroed by k_hist).  k_hist
+// writes each chunk's first-level counts l1h[g][b1] when the first pass has
+// fewer bits than the partition.
 // Pass 2 derives its cursors from sb_start and colsum itself.
-__global__ __launch_bounds__(1024) void k_colsums(const uint32_t* __restrict__ part, int G, int bits, int bits2,
-                                                  int W, uint32_t* __restrict__ colsum, uint32_t* __restrict__ totals,
-                                                  uint32_t* __restrict__ l1h)
+__global__ __launch_bounds__(1024) void k_colsums(const uint32_t* __restrict__ part, int G, int bits, int W,
+                                                  uint32_t* __restrict__ colsum, uint32_t* __restrict__ totals)
 {
     constexpr int B = 1024;
-    const int nbins = 1 << bits, P1 = 1 << (bits - bits2);
-    const int ncb = (nbins + B - 1) / B, nA = ncb * W;
-    if ((int)blockIdx.x < nA) {
+    const int nbins = 1 << bits;
+    const int ncb = (nbins + B - 1) / B;
+    {
         const int cb = blockIdx.x % ncb, w = blockIdx.x / ncb;
         const int c = cb * B + threadIdx.x;
         const int g0 = (int)((int64_t)w * G / W), g1 = (int)((int64_t)(w + 1) * G / W);
@@ -169,15 +181,6 @@ __global__ __launch_bounds__(1024) void k_colsums(const uint32_t* __restrict__ p
             for (int g = g0; g < g1; g++) sum += part[(size_t)g * nbins + c];
             colsum[(size_t)w * nbins + c] = sum;
             if (sum) atomicAdd(&totals[c], sum);
-        }
-    } else {
-        const int t = (blockIdx.x - nA) * B + threadIdx.x;
-        if (t < P1 * G) {
-            const int g = t / P1, b1 = t % P1;
-            const uint32_t* src = part + (size_t)g * nbins + ((size_t)b1 << bits2);
-            uint32_t sum = 0;
-            for (int b2 = 0; b2 < (1 << bits2); b2++) sum += src[b2];
-            l1h[t] = sum;
         }
     }
 }
@@ -193,9 +196,12 @@ __global__ __launch_bounds__(1024) void k_starts(const uint32_t* __restrict__ to
     const int b0 = min((int)threadIdx.x * per, nbins), b1 = min(b0 + per, nbins);
     uint32_t v[MAXPER];
     uint64_t sum = 0;
+    // clamped loads, all in flight together, masked afterwards
+#pragma unroll
+    for (int i = 0; i < MAXPER; i++) v[i] = totals[min(b0 + i, nbins - 1)];
 #pragma unroll
     for (int i = 0; i < MAXPER; i++) {
-        v[i] = (b0 + i < b1) ? totals[b0 + i] : 0u;
+        v[i] = (b0 + i < b1) ? v[i] : 0u;
         sum += v[i];
     }
     uint64_t tot;
@@ -658,49 +664,73 @@ __device__ __forceinline__ void sb_sizes(const uint64_t* r_start, const uint64_t
     ns = s_cnt ? (uint64_t)s_cnt[sb] : (s_start[sb + 1] - s0);
 }
 
-// One workgroup: thread t owns a run of at most 32 sub-buckets; counts, one
-// block scan into item_start (item_start[nbins] = total), then the items.
-__global__ __launch_bounds__(1024) void k_plan(const uint64_t* __restrict__ r_start,
-                                               const uint64_t* __restrict__ s_start,
-                                               const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile,
-                                               uint32_t slice, uint32_t max_slices,
-                                               uint64_t* __restrict__ item_start, uint64_t item_cap,
-                                               HeavyItem* __restrict__ items, uint64_t* __restrict__ err)
+__global__ void k_plan_count(const uint64_t* __restrict__ r_start, const uint64_t* __restrict__ s_start,
+                             const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile,
+                             uint32_t slice, uint32_t max_slices, uint32_t* __restrict__ n_items)
 {
+    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= nbins) return;
+    uint64_t r0, nr, s0, ns;
+    sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
+    uint32_t c, sl;
+    n_items[sb] = plan_items(nr, ns, tile, slice, max_slices, c, sl);
+}
+
+// One workgroup: thread t owns a run of at most 32 sub-buckets; one block scan
+// of the counts into item_start (item_start[nbins] = total), then the items of
+// the (few) split sub-buckets.
+__global__ __launch_bounds__(1024) void k_plan_fill(const uint64_t* __restrict__ r_start,
+                                                    const uint64_t* __restrict__ s_start,
+                                                    const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile,
+                                                    uint32_t slice, uint32_t max_slices,
+                                                    const uint32_t* __restrict__ n_items,
+                                                    uint64_t* __restrict__ item_start, uint64_t item_cap,
+                                                    HeavyItem* __restrict__ items, uint64_t* __restrict__ err)
+{
+    constexpr int MAXPER = 32;
     __shared__ uint64_t wtmp[1024 / WAVE + 1];
     const int per = (nbins + 1023) / 1024;
     const int b0 = min((int)threadIdx.x * per, nbins), b1 = min(b0 + per, nbins);
+    uint32_t v[MAXPER];
     uint64_t sum = 0;
-    for (int sb = b0; sb < b1; sb++) {
-        uint64_t r0, nr, s0, ns;
-        sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
-        uint32_t c, sl;
-        sum += plan_items(nr, ns, tile, slice, max_slices, c, sl);
+    // clamped loads, all in flight together, masked afterwards
+#pragma unroll
+    for (int i = 0; i < MAXPER; i++) v[i] = n_items[min(b0 + i, nbins - 1)];
+#pragma unroll
+    for (int i = 0; i < MAXPER; i++) {
+        v[i] = (b0 + i < b1) ? v[i] : 0u;
+        sum += v[i];
     }
     uint64_t total;
     uint64_t base = block_exclusive_scan<1024, uint64_t>(sum, wtmp, &total);
     if (threadIdx.x == 0) item_start[nbins] = total;
     bool over = false;
-    for (int sb = b0; sb < b1; sb++) {
-        uint64_t r0, nr, s0, ns;
-        sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
-        uint32_t chunks, slices;
-        const uint32_t n = plan_items(nr, ns, tile, slice, max_slices, chunks, slices);
-        item_start[sb] = base;
-        if (base + n > item_cap) {
-            over = true;
-        } else {
-            for (uint32_t c = 0; c < chunks; c++)
-                for (uint32_t q = 0; q < slices; q++) {
-                    HeavyItem it;
-                    it.sb = (uint32_t)sb;
-                    it.chunk = c;
-                    it.slice = q;
-                    it.nslices = slices;
-                    items[base + (uint64_t)c * slices + q] = it;
+#pragma unroll
+    for (int i = 0; i < MAXPER; i++) {
+        const int sb = b0 + i;
+        if (sb < b1) {
+            item_start[sb] = base;
+            if (v[i]) {
+                uint64_t r0, nr, s0, ns;
+                sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
+                uint32_t chunks, slices;
+                plan_items(nr, ns, tile, slice, max_slices, chunks, slices);
+                if (base + v[i] > item_cap) {
+                    over = true;
+                } else {
+                    for (uint32_t c = 0; c < chunks; c++)
+                        for (uint32_t q = 0; q < slices; q++) {
+                            HeavyItem it;
+                            it.sb = (uint32_t)sb;
+                            it.chunk = c;
+                            it.slice = q;
+                            it.nslices = slices;
+                            items[base + (uint64_t)c * slices + q] = it;
+                        }
                 }
+            }
+            base += v[i];
         }
-        base += n;
     }
     if (over) atomicOr((unsigned long long*)err, 4ull);
 }
@@ -839,10 +869,10 @@ hipError_t kernels_init(int max_lds_bytes)
 }
 
 hipError_t launch_hist(const uint32_t* key, uint64_t n_host, const uint64_t* n_dev, int G, int bits,
-                       uint32_t* part, uint32_t* totals, hipStream_t st)
+                       uint32_t* part, uint32_t* totals, int bits2, uint32_t* l1h, hipStream_t st)
 {
     const size_t lds = sizeof(uint32_t) << bits;
-    k_hist<HIST_BLOCK><<<G, HIST_BLOCK, lds, st>>>(key, n_host, n_dev, G, bits, part, totals);
+    k_hist<HIST_BLOCK><<<G, HIST_BLOCK, lds, st>>>(key, n_host, n_dev, G, bits, part, totals, bits2, l1h);
     return hipGetLastError();
 }
 
@@ -859,8 +889,7 @@ hipError_t launch_offsets(const uint32_t* part, int G, int bits, int bits2, int 
 {
     if (bits > 15) return hipErrorInvalidValue;
     const int nbins = 1 << bits, P1 = 1 << (bits - bits2);
-    const int grid = ceil_div_i(nbins, 1024) * W + (bits2 ? ceil_div_i((uint64_t)P1 * G, 1024) : 0);
-    k_colsums<<<grid, 1024, 0, st>>>(part, G, bits, bits2, W, colsum, totals, l1h);
+    k_colsums<<<ceil_div_i(nbins, 1024) * W, 1024, 0, st>>>(part, G, bits, W, colsum, totals);
     k_starts<<<1, 1024, 0, st>>>(totals, nbins, sb_start);
     k_l1_offsets<<<ceil_div_i(P1, 64), 1024, 0, st>>>(bits2 ? l1h : part, G, P1, bits2, sb_start, l1off);
     return hipGetLastError();
@@ -891,13 +920,16 @@ static int log2i(int x)
     return l;
 }
 
-static size_t scatter_lds_tile(int nbk, int tile)
+// One-workgroup-per-CU launches have the CU's LDS to themselves: whole 128-byte
+// lines of keys and of values per carried chunk up to 256 buckets.
+static int scatter_carry_big(int nbk) { return nbk <= 256 ? 32 : scatter_carry(nbk); }
+
+static size_t scatter_lds_tile(int nbk, int tile, int cs)
 {
-    const int cs = scatter_carry(nbk);
     return (size_t)nbk * 8 + (size_t)nbk * cs * 8 + (size_t)tile * 8 + (size_t)nbk * 20 + (SC_BLOCK / WAVE + 1) * 4;
 }
 
-size_t scatter_lds_bytes(int nbk) { return scatter_lds_tile(nbk, SC_BLOCK * SC_ITEMS); }
+size_t scatter_lds_bytes(int nbk) { return scatter_lds_tile(nbk, SC_BLOCK * SC_ITEMS, scatter_carry(nbk)); }
 
 hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* okey, uint32_t* oval,
                           uint64_t out_cap, uint64_t n_host, const uint64_t* n_dev, int G, int mode,
@@ -908,11 +940,12 @@ hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* 
     const int grid = (mode == 0) ? G : (1 << p1) * W2;
     const int cs = log2i(scatter_carry(nbk));
     const size_t lds = std::max(scatter_lds_bytes(nbk), min_lds);
-    const size_t lds_big = std::max(scatter_lds_tile(nbk, SC_BLOCK * SC_BIG_ITEMS), min_lds);
+    const int cs_big = scatter_carry_big(nbk);
+    const size_t lds_big = std::max(scatter_lds_tile(nbk, SC_BLOCK * SC_BIG_ITEMS, cs_big), min_lds);
     if (wg != 512 && min_lds > 0 && lds_big <= g_max_lds) {
-        // one workgroup per CU: twice the tile, twice the loads in flight
+        // one workgroup per CU: twice the tile, twice the loads in flight, whole-line carries
         k_scatter<SC_BLOCK, SC_BIG_ITEMS><<<grid, SC_BLOCK, lds_big, st>>>(
-            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, cs, l1off, sb_start, colsum,
+            ikey, ival, okey, oval, out_cap, n_host, n_dev, G, mode, bits, p1, W2, log2i(cs_big), l1off, sb_start, colsum,
             err);
     } else if (wg == 512) {
         // same tile (4096 tuples), half the threads: more workgroups per CU
@@ -971,11 +1004,13 @@ hipError_t launch_plan(const uint64_t* r_start, const uint64_t* s_start, const u
                        uint64_t* item_start, uint64_t* partials, HeavyItem* items, uint64_t item_cap,
                        uint64_t* err, hipStream_t st)
 {
-    (void)counts;
     (void)partials;
     if (bits > 15) return hipErrorInvalidValue;
-    k_plan<<<1, 1024, 0, st>>>(r_start, s_start, s_cnt, 1 << bits, tile, slice, max_slices, item_start, item_cap,
-                               items, err);
+    const int nbins = 1 << bits;
+    k_plan_count<<<ceil_div_i(nbins, 256), 256, 0, st>>>(r_start, s_start, s_cnt, nbins, tile, slice, max_slices,
+                                                         counts);
+    k_plan_fill<<<1, 1024, 0, st>>>(r_start, s_start, s_cnt, nbins, tile, slice, max_slices, counts, item_start,
+                                    item_cap, items, err);
     return hipGetLastError();
 }
 

@@ -269,3 +269,67 @@ def test_partition_kernels(gpu_ctx, n, kw):
     hb = radix_hash_np(hk).astype(np.uint64) >> np.uint64(32 - bits)
     pos = np.arange(n, dtype=np.uint64)
     assert np.all((start[hb.astype(np.int64)] <= pos) & (pos < start[hb.astype(np.int64) + 1]))
+
+
+@pytest.mark.parametrize("n,kw", [
+    (40_000_000, dict(partition_size=512, radix_bits=14)),   # 128 + 128 buckets
+    (36_000_000, dict(partition_size=256, radix_bits=15)),   # 256 + 128 buckets
+])
+def test_partition_large_relations(gpu_ctx, n, kw):
+    """Relations of 32M tuples or more are scattered by one workgroup per CU with
+    8192-tuple tiles and whole-line carries: the same partition properties as
+    above, checked on the device."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(n)
+    key = torch.randint(-2**31, 2**31 - 1, (n,), dtype=torch.int32, device="cuda", generator=g)
+    key[: n // 10] = key[0]                                   # a heavy key
+    val = torch.arange(n, dtype=torch.int32, device="cuda")
+    p = bj.default_params(**kw)
+    info = bj.plan(bj.default_params(r_size=n, s_size=n, **kw))
+    bits = info.bits1 + info.bits2
+    ok, ov = dev_empty(n), dev_empty(n)
+    st = torch.zeros((1 << bits) + 1, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    print(f"[partition_large] n={n} {kw}", flush=True)
+    gpu_ctx.partition(p, key, val, n, ok, ov, st)
+    torch.cuda.synchronize()
+    ov64 = ov[:n].to(torch.int64)
+    assert torch.equal(torch.sort(ov64).values, torch.arange(n, device="cuda"))   # a permutation
+    assert torch.equal(ok[:n], key[ov64])                                         # keys travel with values
+    mul = torch.tensor(0x9E3779B1, dtype=torch.int64, device="cuda")
+    h_in = ((key.to(torch.int64) & 0xFFFFFFFF) * mul & 0xFFFFFFFF) >> (32 - bits)
+    hist = torch.bincount(h_in, minlength=1 << bits)
+    assert torch.equal(st[1:] - st[:-1], hist)                                    # bucket sizes
+    assert int(st[0]) == 0 and int(st[-1]) == n
+    h_out = ((ok[:n].to(torch.int64) & 0xFFFFFFFF) * mul & 0xFFFFFFFF) >> (32 - bits)
+    pos = torch.arange(n, device="cuda")
+    assert bool(((st[h_out] <= pos) & (pos < st[h_out + 1])).all())             # each in its bucket
+
+
+@pytest.mark.parametrize("kw", [
+    dict(partition_size=512, radix_bits=14, use_bf=1, bf_bits_per_key=10.0, num_hashes=3),
+    dict(partition_size=512, radix_bits=14),
+])
+def test_large_join_rows(gpu_ctx, kw):
+    """R = S = 48M (the one-workgroup-per-CU scatter path), 10% match.  R's keys
+    are distinct, so the rows are exactly determined by the matching S
+    positions: the s_val set must be {j : S_key[j] < 2^31} (the generator's
+    matches), and every row must satisfy S_key[s_val] == key == R_key[r_val]."""
+    import torch
+    n = 48_000_000
+    m = O.match_count(n, 0.1)
+    dk, dv, sk, sv = (dev_empty(n) for _ in range(4))
+    gpu_ctx.gen_build(42, n, dk, dv)
+    gpu_ctx.gen_probe(42, 42 ^ 0xDEADBEEF, n, n, m, sk, sv)
+    torch.cuda.synchronize()
+    p = bj.default_params(r_size=n, s_size=n, inputs_on_device=1, **kw)
+    print(f"[large_join] n={n} {kw}", flush=True)
+    res = gpu_ctx.join(p, dk, dv, sk, sv)
+    assert res.match_count == m and res.rows_written == m
+    k, a, b = gpu_ctx.rows_to_host(res)
+    skh, dkh = to_host(sk, n), to_host(dk, n)
+    expect_s = np.nonzero(skh < np.uint32(1 << 31))[0]         # keys below 2^31 match
+    assert len(expect_s) == m
+    assert np.array_equal(np.sort(b.astype(np.int64)), expect_s)
+    assert np.array_equal(skh[b], k)
+    assert np.array_equal(dkh[a], k)
