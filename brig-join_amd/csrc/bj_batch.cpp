@@ -348,6 +348,10 @@ int enqueue(bj_batch* b, hipStream_t st)
     uint64_t* work = ctr + (size_t)n * JC_N;
     uint64_t* err = work + 2;
     const int num_cu = b->c->num_cu;
+    // second passes over many tuples scatter with one workgroup per CU (as
+    // bj_join does for large relations); first passes keep two per CU, which
+    // hides their per-chunk cursor sums better
+    const bool big_r = L.tot_r >= (32ull << 20), big_s = L.tot_s >= (32ull << 20);
 
     BJ_HIP(launch_zero_u64(ctr, (int)ctr_words(n), st));
     // R: one-level partition of every join into the R arena
@@ -358,10 +362,11 @@ int enqueue(bj_batch* b, hipStream_t st)
     BJ_HIP(launch_batch_scatter(B, 0, L.nt_r, nullptr, nullptr, jc, P<uint32_t>(b->cur_rel),
                                 P<uint64_t>(b->start_r), P<uint32_t>(b->r_key), P<uint32_t>(b->r_val),
                                 b->r_key.bytes / 4, P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
-                                b->tmp_key.bytes / 4, err, st));
+                                b->tmp_key.bytes / 4, err, false, st));
     BJ_HIP(launch_batch_scatter2(B, 0, L.nt_r2, P<uint32_t>(b->cur_rel), P<uint64_t>(b->start_r),
                                  P<uint32_t>(b->tot_r), P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
-                                 P<uint32_t>(b->r_key), P<uint32_t>(b->r_val), b->r_key.bytes / 4, err, st));
+                                 P<uint32_t>(b->r_key), P<uint32_t>(b->r_val), b->r_key.bytes / 4, err, big_r,
+                                 st));
     // Bloom joins: filter from the partitioned R, then S -> S'
     if (L.any_bf) {
         BJ_HIP(launch_batch_bloom_build(B, L.nseg, L.max_wps, P<uint32_t>(b->r_key), P<uint64_t>(b->start_r),
@@ -377,10 +382,11 @@ int enqueue(bj_batch* b, hipStream_t st)
     BJ_HIP(launch_batch_scatter(B, 1, L.nt_s, P<uint32_t>(b->sp_key), P<uint32_t>(b->sp_val), jc,
                                 P<uint32_t>(b->cur_rel), P<uint64_t>(b->start_s), P<uint32_t>(b->s_key),
                                 P<uint32_t>(b->s_val), b->s_key.bytes / 4, P<uint32_t>(b->tmp_key),
-                                P<uint32_t>(b->tmp_val), b->tmp_key.bytes / 4, err, st));
+                                P<uint32_t>(b->tmp_val), b->tmp_key.bytes / 4, err, false, st));
     BJ_HIP(launch_batch_scatter2(B, 1, L.nt_s2, P<uint32_t>(b->cur_rel), P<uint64_t>(b->start_s),
                                  P<uint32_t>(b->tot_s), P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
-                                 P<uint32_t>(b->s_key), P<uint32_t>(b->s_val), b->s_key.bytes / 4, err, st));
+                                 P<uint32_t>(b->s_key), P<uint32_t>(b->s_val), b->s_key.bytes / 4, err, big_s,
+                                 st));
     // vlsplit plan and the persistent build + probe, one launch per tile class
     BJ_HIP(launch_batch_plan(B, L.nsb, P<uint64_t>(b->start_r), P<uint32_t>(b->tot_r), P<uint64_t>(b->start_s),
                              P<uint32_t>(b->tot_s), B_SLICE, B_MAX_SLICES, P<uint32_t>(b->plan_cnt),

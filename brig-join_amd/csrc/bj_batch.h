@@ -70,13 +70,14 @@ hipError_t launch_batch_binscan(const BatchDev& B, int side, const uint32_t* tot
 hipError_t launch_batch_scatter(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* sp_key,
                                 const uint32_t* sp_val, const uint64_t* jc, const uint32_t* cur_rel,
                                 const uint64_t* start, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
-                                uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err,
+                                uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err, bool big,
                                 hipStream_t st);
-// Second pass of two-pass joins: scratch arena -> final arena.
+// Second pass of two-pass joins: scratch arena -> final arena.  big: one
+// workgroup per CU with 8192-tuple tiles (for launches over many tuples).
 hipError_t launch_batch_scatter2(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* cur_rel,
                                  const uint64_t* start, const uint32_t* cnt, const uint32_t* tkey,
                                  const uint32_t* tval, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
-                                 uint64_t* err, hipStream_t st);
+                                 uint64_t* err, bool big, hipStream_t st);
 hipError_t launch_batch_bloom_build(const BatchDev& B, uint32_t nseg, uint32_t max_wps, const uint32_t* rkey,
                                     const uint64_t* r_start, const uint32_t* r_cnt, uint64_t* filter,
                                     hipStream_t st);

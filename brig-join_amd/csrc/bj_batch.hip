@@ -163,8 +163,16 @@ __global__ __launch_bounds__(1024) void kb_binscan(BatchDev B, int side, const u
 // start[B << b2] + sum_j cur_rel[c][(B << b2) + j].  One-pass joins (b1 == bits)
 // write the final arena, two-pass joins the scratch arena at the same
 // positions.
+// ITEMS 4: two workgroups per CU, 4096-tuple tiles; ITEMS 8: one per CU (large
+// launches), 8192-tuple tiles and whole-line carries up to 256 buckets.
+__device__ __forceinline__ int kb_carry_log2(int nbk, bool big)
+{
+    if (big && nbk <= 256) return 5;
+    return nbk <= 512 ? 4 : (nbk <= 1024 ? 3 : 2);  // scatter_carry
+}
+
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void kb_scatter(BatchDev B, int side, const uint32_t* __restrict__ sp_key,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ITEMS == 4 ? 8 : 4, 8))) void kb_scatter(BatchDev B, int side, const uint32_t* __restrict__ sp_key,
                                                     const uint32_t* __restrict__ sp_val,
                                                     const uint64_t* __restrict__ jc,
                                                     const uint32_t* __restrict__ cur_rel,
@@ -181,14 +189,34 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     const uint32_t c = t - px[b];
     const int b2 = J.bits - J.b1;
     const int nbk = 1 << J.b1;
-    const int cs_log2 = nbk <= 512 ? 4 : (nbk <= 1024 ? 3 : 2);  // scatter_carry
+    const int cs_log2 = kb_carry_log2(nbk, ITEMS == 8);
     const ScatterLds L = scatter_lds<BLOCK, ITEMS>(lds64, nbk, cs_log2);
     const uint32_t* cr = cur_rel + (side ? J.part_s : J.part_r) + ((size_t)c << J.bits);
     const uint64_t* st = start + J.sb_base;
-    for (int j = threadIdx.x; j < nbk; j += BLOCK) {
-        uint64_t cu = st[(size_t)j << b2];
-        for (int q = 0; q < (1 << b2); q++) cu += cr[((size_t)j << b2) + q];
-        L.cur[j] = cu;
+    // cursor of bucket j: st[j << b2] + its 2^b2 entries of cr, summed by one wave
+    // (coalesced across lanes) or, for buckets narrower than a wave, by aligned
+    // lane segments
+    {
+        const int seg = 1 << b2, wv = threadIdx.x / WAVE;
+        const uint32_t lane = lane_id();
+        constexpr int NWV = BLOCK / WAVE;
+        if (seg >= WAVE) {
+            for (int j = wv; j < nbk; j += NWV) {
+                uint32_t acc = 0;
+                for (int q = (int)lane; q < seg; q += WAVE) acc += cr[((size_t)j << b2) + q];
+                acc = wave_sum<uint32_t>(acc);
+                if (lane == 0) L.cur[j] = st[(size_t)j << b2] + acc;
+            }
+        } else {
+            const int per = WAVE / seg;  // buckets per wave step
+            for (int j0 = wv * per; j0 < nbk; j0 += NWV * per) {
+                const size_t idx = ((size_t)j0 << b2) + lane;
+                uint32_t v = (idx < ((size_t)nbk << b2)) ? cr[idx] : 0u;
+                for (int o = 1; o < seg; o <<= 1) v += __shfl_xor(v, o, WAVE);
+                const int j = j0 + (int)(lane >> b2);
+                if ((lane & (uint32_t)(seg - 1)) == 0 && j < nbk) L.cur[j] = st[(size_t)j << b2] + v;
+            }
+        }
     }
     const uint32_t *key, *val;
     uint64_t n;
@@ -207,7 +235,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
 // (w2 groups per bucket) and scatters it by the remaining b2 bits.  Its range
 // and its sub-bucket cursors come from cur_rel at chunks g0 and g1.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void kb_scatter2(BatchDev B, int side, const uint32_t* __restrict__ cur_rel,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ITEMS == 4 ? 8 : 4, 8))) void kb_scatter2(BatchDev B, int side, const uint32_t* __restrict__ cur_rel,
                                                      const uint64_t* __restrict__ start,
                                                      const uint32_t* __restrict__ cnt,
                                                      const uint32_t* __restrict__ tkey,
@@ -228,7 +256,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     const int nbk = 1 << b2;
     const int G = (int)side_chunks(J, side);
     const int g0 = (int)(((int64_t)w * G) / W2), g1 = (int)(((int64_t)(w + 1) * G) / W2);
-    const int cs_log2 = nbk <= 512 ? 4 : (nbk <= 1024 ? 3 : 2);
+    const int cs_log2 = kb_carry_log2(nbk, ITEMS == 8);
     const ScatterLds L = scatter_lds<BLOCK, ITEMS>(lds64, nbk, cs_log2);
     const size_t f0 = (size_t)bb << b2;  // first sub-bucket of the bucket, within the join
     const uint32_t* base = cur_rel + (side ? J.part_s : J.part_r);
@@ -421,12 +449,17 @@ static hipError_t set_lds_b(const void* fn, size_t bytes)
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(bytes > stat ? bytes - stat : 0));
 }
 
+static size_t g_max_lds_b = 64 * 1024;
+
 hipError_t batch_kernels_init(int max_lds_bytes)
 {
     hipError_t e;
     if ((e = set_lds_b((const void*)kb_hist<BH_BLOCK>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds_b((const void*)kb_scatter<BS_BLOCK, BS_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
     if ((e = set_lds_b((const void*)kb_scatter2<BS_BLOCK, BS_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds_b((const void*)kb_scatter<BS_BLOCK, 2 * BS_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
+    if ((e = set_lds_b((const void*)kb_scatter2<BS_BLOCK, 2 * BS_ITEMS>, max_lds_bytes)) != hipSuccess) return e;
+    g_max_lds_b = (size_t)max_lds_bytes;
     if ((e = set_lds_b((const void*)kb_bloom_build<1024>, max_lds_bytes)) != hipSuccess) return e;
     return hipSuccess;
 }
@@ -458,10 +491,16 @@ hipError_t launch_batch_binscan(const BatchDev& B, int side, const uint32_t* tot
 hipError_t launch_batch_scatter(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* sp_key,
                                 const uint32_t* sp_val, const uint64_t* jc, const uint32_t* cur_rel,
                                 const uint64_t* start, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
-                                uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err,
+                                uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err, bool big,
                                 hipStream_t st)
 {
     if (ntasks == 0) return hipSuccess;
+    const size_t lds_big = std::max(scatter_lds_bytes_big(1 << B.max_b1), g_max_lds_b / 2 + 1024);
+    if (big && lds_big <= g_max_lds_b) {
+        kb_scatter<BS_BLOCK, 2 * BS_ITEMS><<<ntasks, BS_BLOCK, lds_big, st>>>(
+            B, side, sp_key, sp_val, jc, cur_rel, start, okey, oval, out_cap, tkey, tval, tmp_cap, err);
+        return hipGetLastError();
+    }
     const size_t lds = scatter_lds_bytes(1 << B.max_b1);
     kb_scatter<BS_BLOCK, BS_ITEMS><<<ntasks, BS_BLOCK, lds, st>>>(B, side, sp_key, sp_val, jc, cur_rel, start,
                                                                   okey, oval, out_cap, tkey, tval, tmp_cap, err);
@@ -471,9 +510,15 @@ hipError_t launch_batch_scatter(const BatchDev& B, int side, uint32_t ntasks, co
 hipError_t launch_batch_scatter2(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* cur_rel,
                                  const uint64_t* start, const uint32_t* cnt, const uint32_t* tkey,
                                  const uint32_t* tval, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
-                                 uint64_t* err, hipStream_t st)
+                                 uint64_t* err, bool big, hipStream_t st)
 {
     if (ntasks == 0) return hipSuccess;
+    const size_t lds_big = std::max(scatter_lds_bytes_big(1 << std::max(1, B.max_b2)), g_max_lds_b / 2 + 1024);
+    if (big && lds_big <= g_max_lds_b) {
+        kb_scatter2<BS_BLOCK, 2 * BS_ITEMS><<<ntasks, BS_BLOCK, lds_big, st>>>(B, side, cur_rel, start, cnt, tkey,
+                                                                             tval, okey, oval, out_cap, err);
+        return hipGetLastError();
+    }
     const size_t lds = scatter_lds_bytes(1 << std::max(1, B.max_b2));
     kb_scatter2<BS_BLOCK, BS_ITEMS><<<ntasks, BS_BLOCK, lds, st>>>(B, side, cur_rel, start, cnt, tkey, tval,
                                                                    okey, oval, out_cap, err);

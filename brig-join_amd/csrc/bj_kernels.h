@@ -42,6 +42,8 @@ hipError_t launch_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* 
                        hipStream_t st);
 int scatter_carry(int nbk);
 size_t scatter_lds_bytes(int nbk);
+// LDS of a 1024-thread, 8192-tuple tile with the carries of scatter_carry_big
+size_t scatter_lds_bytes_big(int nbk);
 // pass 1 (mode 0): p1-bit buckets; pass 2 (mode 1): the remaining bits, W2
 // workgroups per pass-1 bucket, cursors from sb_start and colsum.  The launch reserves at
 // least min_lds bytes of LDS per workgroup (a way to cap workgroups per CU).

@@ -931,6 +931,11 @@ static size_t scatter_lds_tile(int nbk, int tile, int cs)
 
 size_t scatter_lds_bytes(int nbk) { return scatter_lds_tile(nbk, SC_BLOCK * SC_ITEMS, scatter_carry(nbk)); }
 
+size_t scatter_lds_bytes_big(int nbk)
+{
+    return scatter_lds_tile(nbk, SC_BLOCK * SC_BIG_ITEMS, scatter_carry_big(nbk));
+}
+
 hipError_t launch_scatter(const uint32_t* ikey, const uint32_t* ival, uint32_t* okey, uint32_t* oval,
                           uint64_t out_cap, uint64_t n_host, const uint64_t* n_dev, int G, int mode,
                           int bits, int p1, int W2, const uint64_t* l1off, const uint64_t* sb_start,
