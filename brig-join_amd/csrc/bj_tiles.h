@@ -365,22 +365,35 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
 // table by the hash bits below the radix bits (up to two buckets per build
 // tuple): bst[t] is the first slot of bucket t (bst[nbk] = bn), hkey[] holds the
 // keys bucket by bucket and hidx[] their row in the chunk (8 bytes of LDS per
-// build tuple).  A probe key reads two bucket starts, then scans a short
-// contiguous run of keys: no pointer chasing.  The probe side (pn keys)
-// streams through in rounds of BLOCK*PITEMS keys (the next round is loaded
-// while the current one is processed).  Each round counts its matches,
-// reserves its rows with one global atomic after a block scan, then writes
-// them.  Both values of a row are read only for matches: the build side's
-// from its (cache-resident) chunk, the probe side's from its slice.  Rows past
-// out_cap are counted, not written.
+// build tuple).  A probe key reads its bucket's start and end, then scans a
+// short contiguous run of keys: no pointer chasing.
+// The probe side (pn keys) streams through in rounds of BLOCK*PITEMS keys (the
+// next round is loaded while the current one is processed).  Each round counts
+// its matches, reserves its rows with one global atomic after a block scan,
+// then writes them: staged in LDS and stored by consecutive threads when the
+// round has at most STAGE rows, else directly from the lanes.  Both values of a
+// row are read only for matches: the build side's from its (cache-resident)
+// chunk, the probe side's from its slice.  Rows past out_cap are counted, not
+// written.
 template <int BLOCK, int CAP>
 struct ProbeLds {
-    uint32_t hkey[CAP];      // keys by bucket (bucket counts while the table is built)
+    static constexpr int STAGE = 2 * BLOCK;  // rows of one round staged in LDS
+    uint32_t stage[3][STAGE];  // key, R value, S value of the round's rows (first:
+                               // short LDS offsets for the per-row writes)
+    uint32_t hkey[CAP];        // keys by bucket (bucket counts while the table is built)
     uint16_t hidx[CAP];
-    uint16_t bst[CAP + 2];   // bucket starts, CAP buckets at most
+    uint16_t bst[CAP + 2];     // bucket starts, CAP buckets at most
     uint32_t wtmp[BLOCK / WAVE + 1];
     unsigned long long s_base;
 };
+
+// 64-bit value that is the same in every lane, in scalar registers
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 
 template <int BLOCK, int CAP, int PITEMS>
 __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32_t* __restrict__ bkp,
@@ -393,7 +406,9 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
 {
     constexpr int ROUND = BLOCK * PITEMS;
     constexpr int BPT = CAP / BLOCK;  // build tuples per thread
+    constexpr int STAGE = ProbeLds<BLOCK, CAP>::STAGE;
     static_assert(CAP % BLOCK == 0 && CAP <= 32768, "bucket and rank pack into 16 bits each");
+    if (pn == 0) return;  // an empty slice (uniform)
 
     // table geometry: 2^tb buckets, about two per build tuple when the probe
     // side is much longer than the build side (shorter runs per probe key), else
@@ -405,7 +420,10 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= CAP / 2) tb++;
     }
     const int nbk = 1 << tb;
-    const int tshift = 32 - tb;
+    // bucket of a key: hash bits [32 - bits - tb, 32 - bits), just below the
+    // radix bits (a field of width 0 is bucket 0)
+    const uint32_t toff = (uint32_t)(32 - bits - tb);
+    auto bucket = [&](uint32_t key) { return __builtin_amdgcn_ubfe(radix_hash(key), toff, (uint32_t)tb); };
     uint32_t* cnt = L.hkey;  // bucket counts until the keys are placed
     for (int i = threadIdx.x; i < nbk; i += BLOCK) cnt[i] = 0u;
     __syncthreads();
@@ -421,7 +439,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
     for (int u = 0; u < BPT; u++) {
         const int i = u * BLOCK + threadIdx.x;
         if (i < bn) {
-            const uint32_t t = tb ? ((radix_hash(bkey[u]) << bits) >> tshift) : 0u;
+            const uint32_t t = bucket(bkey[u]);
             btr[u] = (atomicAdd(&cnt[t], 1u) << 16) | t;
         }
     }
@@ -452,12 +470,10 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
     }
     __syncthreads();
 
+    // key loads clamped to the last key (pn >= 1): no branches around loads
     uint32_t nk[PITEMS];
 #pragma unroll
-    for (int it = 0; it < PITEMS; it++) {
-        const uint32_t idx = (uint32_t)it * BLOCK + threadIdx.x;
-        nk[it] = (idx < pn) ? pkp[idx] : 0u;
-    }
+    for (int it = 0; it < PITEMS; it++) nk[it] = pkp[min((uint32_t)it * BLOCK + threadIdx.x, pn - 1u)];
     for (uint32_t base = 0; base < pn; base += ROUND) {
         uint32_t kk[PITEMS], qs[PITEMS], qe[PITEMS];
 #pragma unroll
@@ -465,27 +481,39 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         // the next round's keys are issued after this round's value loads, so the
         // wait for those values does not cover them
         auto prefetch = [&]() {
+            const uint32_t nb0 = base + ROUND;
+            if (nb0 + ROUND <= pn) {
 #pragma unroll
-            for (int it = 0; it < PITEMS; it++) {
-                const uint32_t idx = base + ROUND + (uint32_t)it * BLOCK + threadIdx.x;
-                if (idx < pn) nk[it] = pkp[idx];
+                for (int it = 0; it < PITEMS; it++) nk[it] = pkp[nb0 + (uint32_t)it * BLOCK + threadIdx.x];
+            } else if (nb0 < pn) {
+#pragma unroll
+                for (int it = 0; it < PITEMS; it++)
+                    nk[it] = pkp[min(nb0 + (uint32_t)it * BLOCK + threadIdx.x, pn - 1u)];
             }
         };
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
-            const uint32_t idx = base + (uint32_t)it * BLOCK + threadIdx.x;
-            const uint32_t t = tb ? ((radix_hash(kk[it]) << bits) >> tshift) : 0u;
-            qs[it] = (idx < pn) ? (uint32_t)L.bst[t] : 0u;
-            qe[it] = (idx < pn) ? (uint32_t)L.bst[t + 1] : 0u;
+            // two 16-bit reads: one 32-bit read at a 2-byte aligned address is
+            // slower on this LDS than the pair (t + 1 <= nbk)
+            const uint32_t t = bucket(kk[it]);
+            qs[it] = L.bst[t];
+            qe[it] = L.bst[t + 1];
+        }
+        if (base + ROUND > pn) {  // last round: keys past pn match nothing
+#pragma unroll
+            for (int it = 0; it < PITEMS; it++)
+                if (base + (uint32_t)it * BLOCK + threadIdx.x >= pn) qe[it] = qs[it];
         }
         // count the matches of every key and note its first matching slot q0
+        // (the run is scanned downwards, so the last match seen is the first)
         uint32_t c = 0, hitmask = 0, multi = 0, q0[PITEMS];
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
             uint32_t m = 0, f = 0;
-            for (uint32_t q = qs[it]; q < qe[it]; q++) {
+            for (uint32_t q = qe[it]; q > qs[it];) {
+                q--;
                 const bool e = L.hkey[q] == kk[it];
-                f = (e && m == 0) ? q : f;
+                f = e ? q : f;
                 m += e;
             }
             q0[it] = f;
@@ -503,17 +531,58 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
             L.s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
         // while the reservation is in flight: the values of both sides of every
         // first match, all loads of the round issued together
-        uint32_t pv[PITEMS], bv[PITEMS];
+        uint32_t pv[PITEMS], bv[PITEMS], hx[PITEMS];
         // The loads are unconditional, lanes without a hit read element 0 (one
         // line per wave): loads under a branch would make the stores below wait
-        // for every load in flight, the next round's keys included.
+        // for every load in flight, the next round's keys included.  The build
+        // rows are read from LDS first, all together (q0 = 0 without a hit).
+#pragma unroll
+        for (int it = 0; it < PITEMS; it++) hx[it] = L.hidx[q0[it]];
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
             const bool hit = (hitmask >> it) & 1u;
             pv[it] = pvp[hit ? base + (uint32_t)it * BLOCK + threadIdx.x : 0u];
-            bv[it] = bvp[hit ? (uint32_t)L.hidx[q0[it]] : 0u];
+            bv[it] = bvp[hx[it]];
         }
         prefetch();
+        if (total <= (uint32_t)STAGE) {  // uniform
+            // rows into LDS at their round offsets, then stored by consecutive
+            // threads; the stage is rewritten only after the next round's scan
+            uint32_t w = pre;
+#pragma unroll
+            for (int it = 0; it < PITEMS; it++) {
+                if (hitmask & (1u << it)) {
+                    const uint32_t pval = pv[it];
+                    L.stage[0][w] = kk[it];
+                    L.stage[1][w] = build_r ? bv[it] : pval;
+                    L.stage[2][w] = build_r ? pval : bv[it];
+                    w++;
+                    if (multi & (1u << it)) {
+                        for (uint32_t q = q0[it] + 1; q < qe[it]; q++) {
+                            if (L.hkey[q] == kk[it]) {
+                                const uint32_t bval = bvp[L.hidx[q]];
+                                L.stage[0][w] = kk[it];
+                                L.stage[1][w] = build_r ? bval : pval;
+                                L.stage[2][w] = build_r ? pval : bval;
+                                w++;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const uint64_t sb = uniform_u64(L.s_base);
+            const uint32_t nw = sb >= out_cap ? 0u : (uint32_t)min<uint64_t>(total, out_cap - sb);
+            uint32_t* ok = o_key + sb;
+            uint32_t* orv = o_rval + sb;
+            uint32_t* osv = o_sval + sb;
+            for (uint32_t j = threadIdx.x; j < nw; j += BLOCK) {
+                ok[j] = L.stage[0][j];
+                orv[j] = L.stage[1][j];
+                osv[j] = L.stage[2][j];
+            }
+            continue;
+        }
         __syncthreads();
         uint64_t w = L.s_base + pre;
 #pragma unroll
