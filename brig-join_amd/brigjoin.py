@@ -74,6 +74,7 @@ class BjResult(ctypes.Structure):
         ("bf_k", ctypes.c_int32), ("bf_words", ctypes.c_uint64), ("tile", ctypes.c_int32),
         ("heavy_items", ctypes.c_uint64), ("bf_stage", ctypes.c_int32),
         ("pass1_bits", ctypes.c_int32), ("s_pass1_bits", ctypes.c_int32),
+        ("s_chunks", ctypes.c_int32),
     ]
 
     def phases(self) -> dict:

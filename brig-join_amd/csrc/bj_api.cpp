@@ -408,9 +408,14 @@ void tl_reset(bj_ctx* c, bool timing)
 
 static hipError_t tl_mark(bj_ctx* c, int& idx)
 {
-    if (c->tl_next >= EV_N) {
-        idx = -1;  // pool exhausted: the slot reads as 0 ms
-        return hipSuccess;
+    if (c->tl_next >= (int)c->ev.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            idx = -1;  // no event: the slot reads as 0 ms
+            return hipSuccess;
+        }
+        c->ev.push_back(e);
     }
     idx = c->tl_next++;
     c->tl_last = idx;
@@ -587,6 +592,7 @@ int bj_create(bj_ctx** out, int device)
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = kernels_init(max_lds);
     if (e == hipSuccess) e = batch_kernels_init(max_lds);
+    c->ev.assign(EV_N, nullptr);
     for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_counters, (CTR_N + 1) * sizeof(uint64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -615,8 +621,14 @@ int bj_destroy(bj_ctx* c)
                       &c->s_start, &c->s_fill,  &c->partials, &c->items,  &c->counters, &c->out_key,
                       &c->out_rval, &c->out_sval, &c->dtmp,  &c->plan_cnt, &c->item_start};
     for (DevBuf* b : bufs) release(*b);
-    for (int i = 0; i < EV_N; i++)
-        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (hipEvent_t ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (int b = 0; b < 2; b++) {
+        if (c->ev_copied[b]) (void)hipEventDestroy(c->ev_copied[b]);
+        if (c->ev_used[b]) (void)hipEventDestroy(c->ev_used[b]);
+    }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    if (c->h_chunk) (void)hipHostFree(c->h_chunk);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -653,6 +665,250 @@ int bj_plan(const bj_params* p, bj_plan_info* out)
     return BJ_OK;
 }
 
+}  // extern "C"
+
+namespace bj {
+
+// Fills the parts of a join's result that come from its plan and counters
+// (h: CTR_N counters; s_prime / heavy: |S'| and the vlsplit work items).
+static void fill_result(bj_ctx* c, const Plan& pl, const uint64_t* h, uint64_t s_prime, uint64_t heavy,
+                        bj_result* out)
+{
+    out->match_count = h[CTR_OUT];
+    out->capacity = pl.cap;
+    out->rows_written = std::min(h[CTR_OUT], pl.cap);
+    out->truncated = h[CTR_OUT] > pl.cap ? 1 : 0;
+    out->s_survivors = s_prime;
+    out->key = P<uint32_t>(c->out_key);
+    out->r_val = P<uint32_t>(c->out_rval);
+    out->s_val = P<uint32_t>(c->out_sval);
+    out->bits1 = pl.bits1;
+    out->bits2 = pl.bits2;
+    out->bf_seg_bits = pl.bf ? pl.geom.seg_bits : 0;
+    out->bf_k = pl.bf ? pl.geom.k : 0;
+    out->bf_words = pl.bf_words;
+    out->tile = pl.tile;
+    out->heavy_items = heavy;
+    out->bf_stage = pl.bf ? pl.bf_stage : 0;
+    out->pass1_bits = pl.p1;
+    out->s_pass1_bits = pl.p1_s;
+}
+
+static void fill_phases(bj_result* out, double h2d_ms, double total_ms)
+{
+    const double* k = out->kernel_ms;
+    out->phase_ms[BJ_PH_H2D] = h2d_ms;
+    out->phase_ms[BJ_PH_R_PART] = k[BJ_K_HIST_R] + k[BJ_K_SCAN_R] + k[BJ_K_SCATTER1_R] + k[BJ_K_SCATTER2_R];
+    out->phase_ms[BJ_PH_BF_BUILD] = k[BJ_K_BF_BUILD];
+    out->phase_ms[BJ_PH_S_FILTER] = k[BJ_K_BF_FILTER];
+    out->phase_ms[BJ_PH_S_PART] = k[BJ_K_HIST_S] + k[BJ_K_SCAN_S] + k[BJ_K_SCATTER1_S] + k[BJ_K_SCATTER2_S];
+    out->phase_ms[BJ_PH_PROBE] = k[BJ_K_PLAN] + k[BJ_K_PROBE];
+    out->phase_ms[BJ_PH_TOTAL] = total_ms;
+}
+
+// The plan of one chunk of S: the full join's radix bits, passes, tile and
+// filter (S is partitioned exactly as R), sizes from the chunk, the full
+// join's row capacity.
+static int chunk_plan(const bj_params& p, const Plan& full, uint64_t chunk, Plan& cp)
+{
+    bj_params q = p;
+    q.s_size = (int64_t)chunk;
+    q.radix_bits = full.bits;
+    q.partition_size = 1 << full.bits1;
+    q.second_partition_size = 1 << full.bits2;
+    q.pass1_bits = full.p1;
+    q.s_pass1_bits = full.p1_s;
+    q.tile = full.tile;
+    if (full.bf) q.bf_stage = full.bf_stage;
+    int rc = make_plan(q, cp);
+    if (rc) return rc;
+    if (cp.bits != full.bits || cp.p1 != full.p1 || cp.p1_s != full.p1_s ||
+        (full.bf && (cp.geom.seg_bits != full.geom.seg_bits || cp.geom.wps != full.geom.wps ||
+                     cp.geom.k != full.geom.k || cp.bf_stage != full.bf_stage)))
+        return fail(BJ_ERR_INTERNAL, "chunk plan differs from the join's");
+    cp.cap = full.cap;
+    return BJ_OK;
+}
+
+// Device bytes a context holds.
+static uint64_t held_bytes(bj_ctx* c)
+{
+    uint64_t t = 0;
+    const DevBuf* bufs[] = {&c->in_rk,  &c->in_rv,  &c->in_sk,   &c->in_sv,   &c->r_key,  &c->r_val,
+                            &c->s_key,  &c->s_val,  &c->tmp_key, &c->tmp_val, &c->sp_key, &c->sp_val,
+                            &c->filter, &c->part,   &c->totals,  &c->l1h,     &c->l1off,  &c->colsum,
+                            &c->r_start, &c->s_start, &c->s_fill, &c->partials, &c->items, &c->counters,
+                            &c->out_key, &c->out_rval, &c->out_sval, &c->dtmp, &c->plan_cnt, &c->item_start};
+    for (const DevBuf* b : bufs) t += b->bytes;
+    return t;
+}
+
+// s_chunks = 0: the fewest chunks (powers of two up to 1024) whose workspace,
+// with the staged host inputs, fits 90% of the device memory this context can
+// use (free memory plus what it already holds).
+static int auto_s_chunks(bj_ctx* c, const bj_params& p, const Plan& full, int& chunks)
+{
+    size_t fr = 0, tot = 0;
+    BJ_HIP(hipMemGetInfo(&fr, &tot));
+    const uint64_t budget = (uint64_t)((double)(fr + held_bytes(c)) * 0.9);
+    for (int k = 1; k <= 1024; k *= 2) {
+        Plan cp;
+        const uint64_t chunk = std::max<uint64_t>((full.ns + k - 1) / k, 1);
+        int rc = BJ_OK;
+        if (k == 1) cp = full;
+        else rc = chunk_plan(p, full, chunk, cp);
+        if (rc) return rc;
+        uint64_t ws = 0;
+        ensure_workspace(nullptr, cp, &ws);
+        if (!p.inputs_on_device) ws += full.nr * 8 + (k == 1 ? full.ns * 8 : 2 * chunk * 8);
+        if (ws <= budget) {
+            chunks = k;
+            return BJ_OK;
+        }
+    }
+    return fail(BJ_ERR_NOMEM, "no split of S into up to 1024 chunks fits the device memory");
+}
+
+static int grow_pinned(uint64_t*& h, uint64_t& n, uint64_t need)
+{
+    if (n >= need) return BJ_OK;
+    if (h) (void)hipHostFree(h);
+    h = nullptr;
+    n = 0;
+    BJ_HIP(hipHostMalloc((void**)&h, need * sizeof(uint64_t), hipHostMallocDefault));
+    n = need;
+    return BJ_OK;
+}
+
+// S in chunks (the reference's s_chunks loop, src/hash_kernel.cu:646-650,
+// 910-931): R is partitioned and its filter built once; chunk i of S is
+// filtered, partitioned with R's radix bits and probed, its rows appended
+// through the one row counter.  Host chunks are copied on a second stream into
+// two device buffers, chunk i + 1 while chunk i is joined.
+static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int chunks, const uint32_t* r_key,
+                         const uint32_t* r_val, const uint32_t* s_key, const uint32_t* s_val, bj_result* out)
+{
+    const uint64_t ns = full.ns;
+    const uint64_t chunk = (ns + (uint64_t)chunks - 1) / (uint64_t)chunks;
+    const int nch = (int)((ns + chunk - 1) / chunk);  // chunks holding tuples
+    Plan cp;
+    int rc = chunk_plan(*p, full, chunk, cp);
+    if (rc) return rc;
+    if ((rc = ensure_workspace(c, cp))) return rc;
+    hipStream_t st = c->stream;
+    const bool host = !p->inputs_on_device;
+    const bool timing = p->collect_phase_times != 0;
+    if ((rc = grow_pinned(c->h_chunk, c->h_chunk_n, 2 * (uint64_t)nch))) return rc;
+    if (host) {
+        if (!c->copy_stream) BJ_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        for (int b = 0; b < 2; b++) {
+            if (!c->ev_copied[b]) BJ_HIP(hipEventCreateWithFlags(&c->ev_copied[b], hipEventDisableTiming));
+            if (!c->ev_used[b]) BJ_HIP(hipEventCreateWithFlags(&c->ev_used[b], hipEventDisableTiming));
+        }
+        if ((rc = ensure(c->in_rk, full.nr * 4)) || (rc = ensure(c->in_rv, full.nr * 4)) ||
+            (rc = ensure(c->in_sk, 2 * chunk * 4)) || (rc = ensure(c->in_sv, 2 * chunk * 4)))
+            return rc;
+    }
+
+    tl_reset(c, timing);
+    const bool h2d_ev = host && timing;
+    if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D0], st));
+    if (host) {
+        if (full.nr) {
+            BJ_HIP(hipMemcpyAsync(c->in_rk.p, r_key, full.nr * 4, hipMemcpyHostToDevice, st));
+            BJ_HIP(hipMemcpyAsync(c->in_rv.p, r_val, full.nr * 4, hipMemcpyHostToDevice, st));
+        }
+        r_key = P<uint32_t>(c->in_rk);
+        r_val = P<uint32_t>(c->in_rv);
+    }
+    if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D1], st));
+    uint64_t* ctr = P<uint64_t>(c->counters);
+    BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
+    if ((rc = enqueue_r(c, cp, r_key, r_val))) return rc;
+    const RSide R{P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start), P<uint64_t>(c->filter)};
+    const int nbins = 1 << cp.bits;
+    uint64_t* item_start = P<uint64_t>(c->item_start);
+
+    // buffer b of the two holds chunks b, b + 2, ...; its next copy waits until
+    // the join of the chunk before has enqueued everything that reads it
+    auto copy_chunk = [&](int i) -> int {
+        const int b = i & 1;
+        const uint64_t off = (uint64_t)i * chunk, n = std::min(chunk, ns - off);
+        if (i >= 2) BJ_HIP(hipStreamWaitEvent(c->copy_stream, c->ev_used[b], 0));
+        BJ_HIP(hipMemcpyAsync(P<uint32_t>(c->in_sk) + b * chunk, s_key + off, n * 4, hipMemcpyHostToDevice,
+                              c->copy_stream));
+        BJ_HIP(hipMemcpyAsync(P<uint32_t>(c->in_sv) + b * chunk, s_val + off, n * 4, hipMemcpyHostToDevice,
+                              c->copy_stream));
+        BJ_HIP(hipEventRecord(c->ev_copied[b], c->copy_stream));
+        return BJ_OK;
+    };
+    static const int s_slots[] = {BJ_K_BF_FILTER, BJ_K_HIST_S,  BJ_K_SCAN_S, BJ_K_SCATTER1_S,
+                                  BJ_K_SCATTER2_S, BJ_K_PLAN, BJ_K_PROBE};
+    constexpr size_t NSL = sizeof(s_slots) / sizeof(s_slots[0]);
+    std::vector<int> s_events;  // per chunk and S-side slot: begin, end
+    if (host && (rc = copy_chunk(0))) return rc;
+    for (int i = 0; i < nch; i++) {
+        const int b = i & 1;
+        const uint64_t off = (uint64_t)i * chunk, n = std::min(chunk, ns - off);
+        const uint32_t* ck = s_key + off;
+        const uint32_t* cv = s_val + off;
+        if (host) {
+            BJ_HIP(hipStreamWaitEvent(st, c->ev_copied[b], 0));
+            ck = P<uint32_t>(c->in_sk) + b * chunk;
+            cv = P<uint32_t>(c->in_sv) + b * chunk;
+        }
+        // per-chunk counters: S', work items, filter tiles (rows and error flags carry over)
+        BJ_HIP(launch_zero_u64(ctr + CTR_SPRIME, 1, st));
+        BJ_HIP(launch_zero_u64(ctr + CTR_WORK, 2, st));
+        BJ_HIP(launch_zero_u64(ctr + CTR_XCD, CTR_N - CTR_XCD, st));
+        c->tl_open = false;  // the chunk's first slot starts after the wait and the clears
+        Plan pi = cp;
+        pi.ns = n;
+        if ((rc = enqueue_s_probe(c, pi, R, ck, cv, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
+                                  P<uint32_t>(c->out_sval), cp.cap)))
+            return rc;
+        BJ_HIP(hipMemcpyAsync(c->h_chunk + 2 * i, ctr + CTR_SPRIME, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        BJ_HIP(hipMemcpyAsync(c->h_chunk + 2 * i + 1, item_start + nbins, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              st));
+        if (host) {
+            BJ_HIP(hipEventRecord(c->ev_used[b], st));
+            if (i + 1 < nch && (rc = copy_chunk(i + 1))) return rc;
+        }
+        if (timing) {
+            for (int sl : s_slots) {
+                s_events.push_back(c->tl_b[sl]);
+                s_events.push_back(c->tl_e[sl]);
+                c->tl_b[sl] = c->tl_e[sl] = -1;
+            }
+        }
+    }
+    BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    BJ_HIP(hipStreamSynchronize(st));
+
+    const uint64_t* h = c->h_counters;
+    if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
+    uint64_t s_prime = 0, heavy = 0;
+    for (int i = 0; i < nch; i++) {
+        s_prime += c->h_chunk[2 * i];
+        heavy += c->h_chunk[2 * i + 1];
+    }
+    fill_result(c, cp, h, full.bf ? s_prime : ns, heavy, out);
+    out->s_chunks = nch;
+    if (timing) {
+        double* k = out->kernel_ms;
+        for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = tl_ms(c, i);
+        for (size_t e = 0; e < s_events.size(); e += 2) k[s_slots[(e / 2) % NSL]] += ev_ms(c, s_events[e], s_events[e + 1]);
+        // the span ends at the last chunk's probe (its end event is the last of the list)
+        const int last_end = s_events.empty() ? -1 : s_events.back();
+        fill_phases(out, h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0, ev_ms(c, c->tl_b[BJ_K_HIST_R], last_end));
+    }
+    return BJ_OK;
+}
+
+}  // namespace bj
+
+extern "C" {
+
 int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t* r_val,
             const uint32_t* s_key, const uint32_t* s_val, bj_result* out)
 {
@@ -663,7 +919,11 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
     if (rc) return rc;
     if ((pl.nr && (!r_key || !r_val)) || (pl.ns && (!s_key || !s_val)))
         return fail(BJ_ERR_INVALID, "null relation pointer");
+    if (p->s_chunks < 0) return fail(BJ_ERR_INVALID, "s_chunks must be >= 0");
     BJ_HIP(hipSetDevice(c->device));
+    int chunks = p->s_chunks;
+    if (chunks == 0 && (rc = auto_s_chunks(c, *p, pl, chunks))) return rc;
+    if (chunks > 1 && pl.ns > 1) return join_s_chunks(c, p, pl, chunks, r_key, r_val, s_key, s_val, out);
     if ((rc = ensure_workspace(c, pl))) return rc;
     hipStream_t st = c->stream;
 
@@ -709,34 +969,12 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
 
     const uint64_t* h = c->h_counters;
     if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
-    out->match_count = h[CTR_OUT];
-    out->capacity = pl.cap;
-    out->rows_written = std::min(h[CTR_OUT], pl.cap);
-    out->truncated = h[CTR_OUT] > pl.cap ? 1 : 0;
-    out->s_survivors = pl.bf ? h[CTR_SPRIME] : pl.ns;
-    out->key = P<uint32_t>(c->out_key);
-    out->r_val = P<uint32_t>(c->out_rval);
-    out->s_val = P<uint32_t>(c->out_sval);
-    out->bits1 = pl.bits1;
-    out->bits2 = pl.bits2;
-    out->bf_seg_bits = pl.bf ? pl.geom.seg_bits : 0;
-    out->bf_k = pl.bf ? pl.geom.k : 0;
-    out->bf_words = pl.bf_words;
-    out->tile = pl.tile;
-    out->heavy_items = h[CTR_N];
-    out->bf_stage = pl.bf ? pl.bf_stage : 0;
-    out->pass1_bits = pl.p1;
-    out->s_pass1_bits = pl.p1_s;
+    fill_result(c, pl, h, pl.bf ? h[CTR_SPRIME] : pl.ns, h[CTR_N], out);
+    out->s_chunks = 1;
     if (p->collect_phase_times) {
         double* k = out->kernel_ms;
         for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = tl_ms(c, i);
-        out->phase_ms[BJ_PH_H2D] = h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0;
-        out->phase_ms[BJ_PH_R_PART] = k[BJ_K_HIST_R] + k[BJ_K_SCAN_R] + k[BJ_K_SCATTER1_R] + k[BJ_K_SCATTER2_R];
-        out->phase_ms[BJ_PH_BF_BUILD] = k[BJ_K_BF_BUILD];
-        out->phase_ms[BJ_PH_S_FILTER] = k[BJ_K_BF_FILTER];
-        out->phase_ms[BJ_PH_S_PART] = k[BJ_K_HIST_S] + k[BJ_K_SCAN_S] + k[BJ_K_SCATTER1_S] + k[BJ_K_SCATTER2_S];
-        out->phase_ms[BJ_PH_PROBE] = k[BJ_K_PLAN] + k[BJ_K_PROBE];
-        out->phase_ms[BJ_PH_TOTAL] = tl_span_ms(c, BJ_K_HIST_R, BJ_K_PROBE);
+        fill_phases(out, h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0, tl_span_ms(c, BJ_K_HIST_R, BJ_K_PROBE));
     }
     return BJ_OK;
 }

@@ -43,8 +43,9 @@ int ceil_log2(uint64_t n);
 // CTR_XCD .. CTR_XCD + 7: tile counters of the filter kernel, one per XCD
 enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_XCD = 8, CTR_N = 16 };
 
-// Events EV_H2D0/1 bracket the host copies; EV_KB .. EV_N - 1 are a pool for
-// the kernel-slot boundaries (tl_begin / tl_end).
+// Events EV_H2D0/1 bracket the host copies; EV_KB .. are a pool for the
+// kernel-slot boundaries (tl_begin / tl_end), EV_N at creation, grown when a
+// join records more (one set of S-side slots per chunk of S).
 enum { EV_H2D0 = 0, EV_H2D1, EV_KB, EV_N = EV_KB + 2 * BJ_NUM_KERNELS };
 
 // The plan of one join (bj_params -> radix bits, filter geometry, sizes).
@@ -86,7 +87,7 @@ struct bj_ctx {
     int max_lds = 64 * 1024;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[bj::EV_N] = {};
+    std::vector<hipEvent_t> ev;
     // timeline of the current join: slot k ran from event tl_b[k] to tl_e[k]
     // (-1: no event); one event serves as the end of a slot and the start of the
     // next, and nothing is recorded when the join does not collect times
@@ -102,6 +103,14 @@ struct bj_ctx {
     bj::DevBuf plan_cnt, item_start;
     bj::DevBuf out_key, out_rval, out_sval, dtmp;
     uint64_t* h_counters = nullptr;  // pinned
+    // S in chunks: copies of host chunks run on copy_stream into two device
+    // buffers (in_sk / in_sv, two chunks each); ev_copied[b] / ev_used[b] order a
+    // buffer's copy before its join and its next copy after that join.  h_chunk
+    // (pinned) receives each chunk's S' size and vlsplit item count.
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_copied[2] = {}, ev_used[2] = {};
+    uint64_t* h_chunk = nullptr;
+    uint64_t h_chunk_n = 0;
     // autotuner table of the batched join (bj_autotune / bj_tune_set)
     std::vector<bj_tune_entry> tune;
 };

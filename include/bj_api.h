@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define BJ_API_VERSION 2
+#define BJ_API_VERSION 3
 
 /* ---- error codes ------------------------------------------------------- */
 #define BJ_OK              0
@@ -99,7 +99,14 @@ typedef struct bj_params {
     int32_t tile;                   /* LDS build tile in tuples (4096 or 8192); 0=auto */
     int32_t inputs_on_device;       /* 1: r_key.. are device pointers; 0: host         */
     int32_t collect_phase_times;    /* 1: fill bj_result.phase_ms                      */
-    int32_t s_chunks;               /* argv[12]; accepted, S is processed in one pass  */
+    int32_t s_chunks;               /* argv[12]: S is joined in this many chunks (the
+                                       reference's s_chunks loop, hash_kernel.cu:910-931):
+                                       R is partitioned and filtered once, each chunk
+                                       of S is filtered, partitioned and probed, rows
+                                       appended; host chunks are copied on a second
+                                       stream while the previous chunk is joined.
+                                       1 = one pass; 0 = the fewest chunks whose
+                                       workspace fits 90% of the device memory        */
     int32_t bf_stage;               /* where the filter drops S tuples: 1 = before S is
                                        partitioned (the reference's order); 2 = after
                                        the first-level pass, one filter slice per
@@ -138,6 +145,7 @@ typedef struct bj_result {
     int32_t  bf_stage;        /* 1 or 2 as in bj_params (0 = Bloom off)              */
     int32_t  pass1_bits;      /* radix bits of the first physical scatter pass       */
     int32_t  s_pass1_bits;    /* the same for S                                      */
+    int32_t  s_chunks;        /* chunks S was joined in (1 = one pass)               */
 } bj_result;
 
 /* ---- the plan a join would use (no device needed) ---------------------- */
