@@ -377,6 +377,11 @@ def main() -> int:
     ap.add_argument("--s-pass1-bits", type=int, default=None, help="override S's first scatter pass bits")
     ap.add_argument("--scatter-wg", type=int, default=None, help="scatter workgroup size (512 or 1024)")
     ap.add_argument("--tile", type=int, default=None, help="probe LDS table capacity (4096 or 8192)")
+    ap.add_argument("--host-inputs", action="store_true",
+                    help="R and S in pinned host memory: the join copies them (value is then the "
+                         "PCIe-inclusive rate, not the HBM-resident one)")
+    ap.add_argument("--s-chunks", type=int, default=None,
+                    help="join S in this many chunks (host chunks copied while the previous one is joined)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE) and exit")
@@ -427,7 +432,18 @@ def main() -> int:
         params = dict(params, scatter_wg=args.scatter_wg)
     if args.tile is not None:
         params = dict(params, tile=args.tile)
-    p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=1, **params)
+    if args.s_chunks is not None:
+        params = dict(params, s_chunks=args.s_chunks)
+    if args.host_inputs:
+        # pinned host copies of the generated relations (the device ones are freed)
+        def pinned(t):
+            h = torch.empty(t.numel(), dtype=t.dtype, pin_memory=True)
+            h.copy_(t)
+            return h
+        rk, rv, sk, sv = (pinned(t) for t in (rk, rv, sk, sv))
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=0 if args.host_inputs else 1, **params)
     bloom = bool(params.get("use_bf"))
 
     for _ in range(args.warmup):
@@ -435,12 +451,14 @@ def main() -> int:
     torch.cuda.synchronize()
     kern_sum = {k: 0.0 for k in bj.KERNELS}
     total_dev = 0.0
+    h2d_sum = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res = ctx.join(p, rk, rv, sk, sv)
         for k, v in res.kernels().items():
             kern_sum[k] += v
         total_dev += res.phase_ms[6]
+        h2d_sum += res.phase_ms[0]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     if res.match_count != m_s:
@@ -478,6 +496,8 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic (device generator: distinct R keys, exact match ratio)",
         "config": {"workload": args.workload, "n_r": n_r, "n_s": n_s, "match_ratio": ratio,
+                   "inputs": "host, pinned (value includes the PCIe copies)" if args.host_inputs else "HBM",
+                   "s_chunks": int(res.s_chunks),
                    "zipf_s": zipf_s, **{k: v for k, v in params.items()},
                    "bits1": int(res.bits1), "bits2": int(res.bits2), "tile": int(res.tile),
                    "bf_stage": int(res.bf_stage), "pass1_bits": int(res.pass1_bits),
@@ -489,7 +509,8 @@ def main() -> int:
         "whole_join": {"alg_bytes": B, "device_ms": round(total_dev / args.steps, 4),
                        "alg_frac_of_8TBps": round(B / (total_dev / args.steps * 1e-3) / 8e12, 4),
                        "matches": m, "s_survivors": ms_sv,
-                       "vlsplit_items": int(res.heavy_items)},
+                       "vlsplit_items": int(res.heavy_items),
+                       "h2d_ms_before_join": round(h2d_sum / args.steps, 4)},
         "kernels": per_kernel,
     }
     if not args.no_cpu_baseline:

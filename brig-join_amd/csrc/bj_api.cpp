@@ -627,8 +627,8 @@ int bj_destroy(bj_ctx* c)
         if (c->ev_copied[b]) (void)hipEventDestroy(c->ev_copied[b]);
         if (c->ev_used[b]) (void)hipEventDestroy(c->ev_used[b]);
     }
+    if (c->ev_r_copied) (void)hipEventDestroy(c->ev_r_copied);
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
-    if (c->h_chunk) (void)hipHostFree(c->h_chunk);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -769,17 +769,6 @@ static int auto_s_chunks(bj_ctx* c, const bj_params& p, const Plan& full, int& c
     return fail(BJ_ERR_NOMEM, "no split of S into up to 1024 chunks fits the device memory");
 }
 
-static int grow_pinned(uint64_t*& h, uint64_t& n, uint64_t need)
-{
-    if (n >= need) return BJ_OK;
-    if (h) (void)hipHostFree(h);
-    h = nullptr;
-    n = 0;
-    BJ_HIP(hipHostMalloc((void**)&h, need * sizeof(uint64_t), hipHostMallocDefault));
-    n = need;
-    return BJ_OK;
-}
-
 // S in chunks (the reference's s_chunks loop, src/hash_kernel.cu:646-650,
 // 910-931): R is partitioned and its filter built once; chunk i of S is
 // filtered, partitioned with R's radix bits and probed, its rows appended
@@ -798,30 +787,36 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
     hipStream_t st = c->stream;
     const bool host = !p->inputs_on_device;
     const bool timing = p->collect_phase_times != 0;
-    if ((rc = grow_pinned(c->h_chunk, c->h_chunk_n, 2 * (uint64_t)nch))) return rc;
     if (host) {
         if (!c->copy_stream) BJ_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         for (int b = 0; b < 2; b++) {
             if (!c->ev_copied[b]) BJ_HIP(hipEventCreateWithFlags(&c->ev_copied[b], hipEventDisableTiming));
             if (!c->ev_used[b]) BJ_HIP(hipEventCreateWithFlags(&c->ev_used[b], hipEventDisableTiming));
         }
+        if (!c->ev_r_copied) BJ_HIP(hipEventCreateWithFlags(&c->ev_r_copied, hipEventDisableTiming));
         if ((rc = ensure(c->in_rk, full.nr * 4)) || (rc = ensure(c->in_rv, full.nr * 4)) ||
             (rc = ensure(c->in_sk, 2 * chunk * 4)) || (rc = ensure(c->in_sv, 2 * chunk * 4)))
             return rc;
     }
 
     tl_reset(c, timing);
+    // R's copy goes first on the copy stream, at the link's full rate; the
+    // chunks of S follow it there while R is partitioned
     const bool h2d_ev = host && timing;
-    if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D0], st));
     if (host) {
+        BJ_HIP(hipEventRecord(c->ev_used[0], st));  // the copies start after earlier work on st
+        BJ_HIP(hipStreamWaitEvent(c->copy_stream, c->ev_used[0], 0));
+        if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D0], c->copy_stream));
         if (full.nr) {
-            BJ_HIP(hipMemcpyAsync(c->in_rk.p, r_key, full.nr * 4, hipMemcpyHostToDevice, st));
-            BJ_HIP(hipMemcpyAsync(c->in_rv.p, r_val, full.nr * 4, hipMemcpyHostToDevice, st));
+            BJ_HIP(hipMemcpyAsync(c->in_rk.p, r_key, full.nr * 4, hipMemcpyHostToDevice, c->copy_stream));
+            BJ_HIP(hipMemcpyAsync(c->in_rv.p, r_val, full.nr * 4, hipMemcpyHostToDevice, c->copy_stream));
         }
+        if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D1], c->copy_stream));
+        BJ_HIP(hipEventRecord(c->ev_r_copied, c->copy_stream));
+        BJ_HIP(hipStreamWaitEvent(st, c->ev_r_copied, 0));
         r_key = P<uint32_t>(c->in_rk);
         r_val = P<uint32_t>(c->in_rv);
     }
-    if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D1], st));
     uint64_t* ctr = P<uint64_t>(c->counters);
     BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
     if ((rc = enqueue_r(c, cp, r_key, r_val))) return rc;
@@ -867,9 +862,9 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
         if ((rc = enqueue_s_probe(c, pi, R, ck, cv, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
                                   P<uint32_t>(c->out_sval), cp.cap)))
             return rc;
-        BJ_HIP(hipMemcpyAsync(c->h_chunk + 2 * i, ctr + CTR_SPRIME, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        BJ_HIP(hipMemcpyAsync(c->h_chunk + 2 * i + 1, item_start + nbins, sizeof(uint64_t), hipMemcpyDeviceToHost,
-                              st));
+        // |S'| and work items of the chunk, summed on the device (a copy to the
+        // host here would queue behind the next chunk's copy)
+        BJ_HIP(launch_add2_u64(ctr + CTR_SPRIME, item_start + nbins, ctr + CTR_ACC_SPRIME, st));
         if (host) {
             BJ_HIP(hipEventRecord(c->ev_used[b], st));
             if (i + 1 < nch && (rc = copy_chunk(i + 1))) return rc;
@@ -887,12 +882,7 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
 
     const uint64_t* h = c->h_counters;
     if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
-    uint64_t s_prime = 0, heavy = 0;
-    for (int i = 0; i < nch; i++) {
-        s_prime += c->h_chunk[2 * i];
-        heavy += c->h_chunk[2 * i + 1];
-    }
-    fill_result(c, cp, h, full.bf ? s_prime : ns, heavy, out);
+    fill_result(c, cp, h, full.bf ? h[CTR_ACC_SPRIME] : ns, h[CTR_ACC_ITEMS], out);
     out->s_chunks = nch;
     if (timing) {
         double* k = out->kernel_ms;

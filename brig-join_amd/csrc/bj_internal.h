@@ -40,8 +40,9 @@ uint64_t splitmix64(uint64_t x);
 PermKeys perm_keys(uint64_t seed);
 int ceil_log2(uint64_t n);
 
-// CTR_XCD .. CTR_XCD + 7: tile counters of the filter kernel, one per XCD
-enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_XCD = 8, CTR_N = 16 };
+// CTR_XCD .. CTR_XCD + 7: tile counters of the filter kernel, one per XCD;
+// CTR_ACC_SPRIME / CTR_ACC_ITEMS: |S'| and vlsplit items summed over the chunks of S
+enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_ACC_SPRIME, CTR_ACC_ITEMS, CTR_XCD = 8, CTR_N = 16 };
 
 // Events EV_H2D0/1 bracket the host copies; EV_KB .. are a pool for the
 // kernel-slot boundaries (tl_begin / tl_end), EV_N at creation, grown when a
@@ -103,14 +104,12 @@ struct bj_ctx {
     bj::DevBuf plan_cnt, item_start;
     bj::DevBuf out_key, out_rval, out_sval, dtmp;
     uint64_t* h_counters = nullptr;  // pinned
-    // S in chunks: copies of host chunks run on copy_stream into two device
-    // buffers (in_sk / in_sv, two chunks each); ev_copied[b] / ev_used[b] order a
-    // buffer's copy before its join and its next copy after that join.  h_chunk
-    // (pinned) receives each chunk's S' size and vlsplit item count.
+    // S in chunks: copies of host inputs run on copy_stream, R's first (then
+    // ev_r_copied), then the chunks of S into two device buffers (in_sk / in_sv,
+    // two chunks each); ev_copied[b] / ev_used[b] order a buffer's copy before
+    // its join and its next copy after that join.
     hipStream_t copy_stream = nullptr;
-    hipEvent_t ev_copied[2] = {}, ev_used[2] = {};
-    uint64_t* h_chunk = nullptr;
-    uint64_t h_chunk_n = 0;
+    hipEvent_t ev_copied[2] = {}, ev_used[2] = {}, ev_r_copied = nullptr;
     // autotuner table of the batched join (bj_autotune / bj_tune_set)
     std::vector<bj_tune_entry> tune;
 };

@@ -19,6 +19,8 @@ struct HeavyItem {
 hipError_t kernels_init(int max_lds_bytes);
 
 hipError_t launch_zero_u64(uint64_t* p, int n, hipStream_t st);
+// acc[0] += *a, acc[1] += *b (one thread): per-chunk counts summed on the device.
+hipError_t launch_add2_u64(const uint64_t* a, const uint64_t* b, uint64_t* acc, hipStream_t st);
 hipError_t launch_gen_r(uint32_t* key, uint32_t* val, uint64_t n, const PermKeys& K, hipStream_t st);
 hipError_t launch_gen_s(uint32_t* key, uint32_t* val, uint64_t n_s, uint64_t n_r, uint64_t m_s,
                         int L, const PermKeys& KR, const PermKeys& KP, const PermKeys& KN,

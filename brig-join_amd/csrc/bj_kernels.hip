@@ -35,6 +35,14 @@ __global__ void k_zero_u64(uint64_t* __restrict__ p, int n)
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
 }
 
+__global__ void k_add2_u64(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, uint64_t* __restrict__ acc)
+{
+    if (threadIdx.x == 0) {
+        acc[0] += *a;
+        acc[1] += *b;
+    }
+}
+
 // ============================================================== generator
 __global__ void k_gen_r(uint32_t* __restrict__ key, uint32_t* __restrict__ val, uint64_t n,
                         PermKeys K)
@@ -800,6 +808,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
 
 // ============================================================== launchers
 static int ceil_div_i(uint64_t a, uint64_t b) { return (int)((a + b - 1) / b); }
+
+hipError_t launch_add2_u64(const uint64_t* a, const uint64_t* b, uint64_t* acc, hipStream_t st)
+{
+    k_add2_u64<<<1, 64, 0, st>>>(a, b, acc);
+    return hipGetLastError();
+}
 
 hipError_t launch_zero_u64(uint64_t* p, int n, hipStream_t st)
 {
