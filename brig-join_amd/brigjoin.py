@@ -20,6 +20,7 @@ BJ_OK = 0
 ERRORS = {-1: "BJ_ERR_INVALID", -2: "BJ_ERR_HIP", -3: "BJ_ERR_NOMEM", -4: "BJ_ERR_IO",
           -5: "BJ_ERR_SIZE", -6: "BJ_ERR_NODEVICE", -7: "BJ_ERR_INTERNAL"}
 PHASES = ["h2d", "r_partition", "bf_build", "s_filter", "s_partition", "probe", "total"]
+PROBE_PHASES = ["items", "clear", "build", "count", "write"]
 KERNELS = ["hist_r", "scan_r", "scatter1_r", "scatter2_r", "bf_build", "bf_filter", "hist_s",
            "scan_s", "scatter1_s", "scatter2_s", "plan", "probe"]
 
@@ -58,7 +59,7 @@ class BjParams(ctypes.Structure):
         ("inputs_on_device", ctypes.c_int32), ("collect_phase_times", ctypes.c_int32),
         ("s_chunks", ctypes.c_int32), ("bf_stage", ctypes.c_int32),
         ("pass1_bits", ctypes.c_int32), ("s_pass1_bits", ctypes.c_int32),
-        ("scatter_wg", ctypes.c_int32),
+        ("scatter_wg", ctypes.c_int32), ("probe_breakdown", ctypes.c_int32),
     ]
 
 
@@ -75,10 +76,14 @@ class BjResult(ctypes.Structure):
         ("heavy_items", ctypes.c_uint64), ("bf_stage", ctypes.c_int32),
         ("pass1_bits", ctypes.c_int32), ("s_pass1_bits", ctypes.c_int32),
         ("s_chunks", ctypes.c_int32),
+        ("probe_phase_ms", ctypes.c_double * 5),
     ]
 
     def phases(self) -> dict:
         return {n: float(self.phase_ms[i]) for i, n in enumerate(PHASES)}
+
+    def probe_phases(self) -> dict:
+        return {n: float(self.probe_phase_ms[i]) for i, n in enumerate(PROBE_PHASES)}
 
     def kernels(self) -> dict:
         return {n: float(self.kernel_ms[i]) for i, n in enumerate(KERNELS)}

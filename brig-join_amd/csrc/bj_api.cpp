@@ -159,6 +159,7 @@ int make_plan(const bj_params& p, Plan& pl)
     if (p.use_bf < 0 || p.use_bf == 11 || p.use_bf == 12 || p.use_bf > 16)
         return fail(BJ_ERR_INVALID, "unsupported use_bf=" + std::to_string(p.use_bf));
     pl.bf = p.use_bf != 0;
+    pl.probe_breakdown = p.probe_breakdown != 0;
 
     int b1 = -1, b2 = -1;
     if (p.partition_size >= 0) {
@@ -517,7 +518,7 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
     BJ_HIP(launch_probe(pl.tile, R.key, R.val, R.start, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val),
                         P<uint64_t>(c->s_start), s_cnt, pl.bits, pl.slice, P<HeavyItem>(c->items),
                         item_start + nbins, pl.item_cap, ctr + CTR_WORK, ctr + CTR_OUT, cap, o_key, o_rval,
-                        o_sval, c->num_cu, st));
+                        o_sval, c->num_cu, pl.probe_breakdown ? P<uint64_t>(c->ptimers) : nullptr, st));
     BJ_HIP(end(BJ_K_PROBE));
     return BJ_OK;
 }
@@ -567,6 +568,7 @@ void bj_params_default(bj_params* p)
     p->pass1_bits = -1;
     p->s_pass1_bits = -1;
     p->scatter_wg = 0;
+    p->probe_breakdown = 0;
 }
 
 int bj_create(bj_ctx** out, int device)
@@ -619,7 +621,7 @@ int bj_destroy(bj_ctx* c)
                       &c->s_key,  &c->s_val,    &c->tmp_key, &c->tmp_val,  &c->sp_key,  &c->sp_val,
                       &c->filter, &c->part,     &c->totals,  &c->l1h,      &c->l1off,   &c->colsum,  &c->r_start,
                       &c->s_start, &c->s_fill,  &c->partials, &c->items,  &c->counters, &c->out_key,
-                      &c->out_rval, &c->out_sval, &c->dtmp,  &c->plan_cnt, &c->item_start};
+                      &c->out_rval, &c->out_sval, &c->dtmp,  &c->plan_cnt, &c->item_start, &c->ptimers};
     for (DevBuf* b : bufs) release(*b);
     for (hipEvent_t ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -668,6 +670,35 @@ int bj_plan(const bj_params* p, bj_plan_info* out)
 }  // extern "C"
 
 namespace bj {
+
+// Probe phase timers: cleared before a join that collects them, turned into
+// milliseconds after it (each phase's share of all workgroups' ticks times the
+// probe kernel's time; chunks of S add to the same timers and times).
+static int probe_timers_begin(bj_ctx* c, const Plan& pl)
+{
+    if (!pl.probe_breakdown) return BJ_OK;
+    const int n = probe_grid(pl.tile, c->num_cu) * PROBE_TIMER_SLOTS;
+    int rc = ensure(c->ptimers, (uint64_t)n * sizeof(uint64_t));
+    if (rc) return rc;
+    BJ_HIP(launch_zero_u64(P<uint64_t>(c->ptimers), n, c->stream));
+    return BJ_OK;
+}
+
+static int probe_timers_end(bj_ctx* c, const Plan& pl, bj_result* out)
+{
+    if (!pl.probe_breakdown) return BJ_OK;
+    const int g = probe_grid(pl.tile, c->num_cu);
+    std::vector<uint64_t> t((size_t)g * PROBE_TIMER_SLOTS);
+    BJ_HIP(hipMemcpy(t.data(), c->ptimers.p, t.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    double sum[PROBE_TIMER_SLOTS] = {}, all = 0.0;
+    for (int w = 0; w < g; w++)
+        for (int p = 0; p < PROBE_TIMER_SLOTS; p++) sum[p] += (double)t[(size_t)w * PROBE_TIMER_SLOTS + p];
+    for (double v : sum) all += v;
+    for (int p = 0; p < PROBE_TIMER_SLOTS; p++)
+        out->probe_phase_ms[p] = all > 0.0 ? out->kernel_ms[BJ_K_PROBE] * sum[p] / all : 0.0;
+    return BJ_OK;
+}
+static_assert(PROBE_TIMER_SLOTS == BJ_NUM_PROBE_PHASES, "probe phase slots");
 
 // Fills the parts of a join's result that come from its plan and counters
 // (h: CTR_N counters; s_prime / heavy: |S'| and the vlsplit work items).
@@ -738,7 +769,8 @@ static uint64_t held_bytes(bj_ctx* c)
                             &c->s_key,  &c->s_val,  &c->tmp_key, &c->tmp_val, &c->sp_key, &c->sp_val,
                             &c->filter, &c->part,   &c->totals,  &c->l1h,     &c->l1off,  &c->colsum,
                             &c->r_start, &c->s_start, &c->s_fill, &c->partials, &c->items, &c->counters,
-                            &c->out_key, &c->out_rval, &c->out_sval, &c->dtmp, &c->plan_cnt, &c->item_start};
+                            &c->out_key, &c->out_rval, &c->out_sval, &c->dtmp, &c->plan_cnt, &c->item_start,
+                            &c->ptimers};
     for (const DevBuf* b : bufs) t += b->bytes;
     return t;
 }
@@ -786,7 +818,7 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
     if ((rc = ensure_workspace(c, cp))) return rc;
     hipStream_t st = c->stream;
     const bool host = !p->inputs_on_device;
-    const bool timing = p->collect_phase_times != 0;
+    const bool timing = p->collect_phase_times != 0 || cp.probe_breakdown;
     if (host) {
         if (!c->copy_stream) BJ_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         for (int b = 0; b < 2; b++) {
@@ -819,6 +851,7 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
     }
     uint64_t* ctr = P<uint64_t>(c->counters);
     BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
+    if ((rc = probe_timers_begin(c, cp))) return rc;
     if ((rc = enqueue_r(c, cp, r_key, r_val))) return rc;
     const RSide R{P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start), P<uint64_t>(c->filter)};
     const int nbins = 1 << cp.bits;
@@ -892,7 +925,7 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
         const int last_end = s_events.empty() ? -1 : s_events.back();
         fill_phases(out, h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0, ev_ms(c, c->tl_b[BJ_K_HIST_R], last_end));
     }
-    return BJ_OK;
+    return probe_timers_end(c, cp, out);
 }
 
 }  // namespace bj
@@ -919,9 +952,10 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
 
     // Kernel-slot events only when the caller collects phase times: each event
     // between two kernels costs the device several microseconds.
-    tl_reset(c, p->collect_phase_times != 0);
+    const bool timing = p->collect_phase_times != 0 || pl.probe_breakdown;
+    tl_reset(c, timing);
     // host inputs: stage them on the device first (timed as BJ_PH_H2D)
-    const bool h2d_ev = !p->inputs_on_device && p->collect_phase_times;
+    const bool h2d_ev = !p->inputs_on_device && timing;
     if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D0], st));
     if (!p->inputs_on_device) {
         if ((rc = ensure(c->in_rk, pl.nr * 4)) || (rc = ensure(c->in_rv, pl.nr * 4)) ||
@@ -944,6 +978,7 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
 
     uint64_t* ctr = P<uint64_t>(c->counters);
     BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
+    if ((rc = probe_timers_begin(c, pl))) return rc;
     // slots without kernels (no filter) keep no events and read as 0 ms
     if ((rc = enqueue_r(c, pl, r_key, r_val))) return rc;
     const RSide R{P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start), P<uint64_t>(c->filter)};
@@ -961,12 +996,12 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
     if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
     fill_result(c, pl, h, pl.bf ? h[CTR_SPRIME] : pl.ns, h[CTR_N], out);
     out->s_chunks = 1;
-    if (p->collect_phase_times) {
+    if (timing) {
         double* k = out->kernel_ms;
         for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = tl_ms(c, i);
         fill_phases(out, h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0, tl_span_ms(c, BJ_K_HIST_R, BJ_K_PROBE));
     }
-    return BJ_OK;
+    return probe_timers_end(c, pl, out);
 }
 
 int bj_partition(bj_ctx* c, const bj_params* p, const uint32_t* key, const uint32_t* val, int64_t n,

@@ -62,6 +62,7 @@ struct Plan {
     uint32_t slice = 65536;
     uint32_t max_slices = 16;
     bool bf = false;
+    bool probe_breakdown = false;  // phase timers in the probe kernel
     int bf_stage = 0;
     BloomGeom geom{0, 8, 1};
     uint64_t bf_words = 0;
@@ -103,6 +104,7 @@ struct bj_ctx {
     bj::DevBuf filter, part, totals, l1h, l1off, colsum, r_start, s_start, s_fill, partials, items, counters;
     bj::DevBuf plan_cnt, item_start;
     bj::DevBuf out_key, out_rval, out_sval, dtmp;
+    bj::DevBuf ptimers;  // probe phase timers (probe_breakdown)
     uint64_t* h_counters = nullptr;  // pinned
     // S in chunks: copies of host inputs run on copy_stream, R's first (then
     // ev_r_copied), then the chunks of S into two device buffers (in_sk / in_sv,

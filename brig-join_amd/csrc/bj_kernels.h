@@ -73,6 +73,11 @@ hipError_t launch_probe(int tile, const uint32_t* rk, const uint32_t* rv, const 
                         const uint32_t* s_cnt, int bits, uint32_t slice, const HeavyItem* items,
                         const uint64_t* n_items, uint64_t item_cap, uint64_t* work_ctr, uint64_t* out_count,
                         uint64_t out_cap, uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, int num_cu,
-                        hipStream_t st);
+                        uint64_t* timers, hipStream_t st);
+// Workgroups of a probe launch (persistent: a fixed number per CU).  With
+// timers, every workgroup adds its phase times (ticks, PT_* order of bj_tiles.h,
+// PT_N = 5 per workgroup) to its slots; the caller clears them.
+int probe_grid(int tile, int num_cu);
+constexpr int PROBE_TIMER_SLOTS = 5;
 
 }  // namespace bj

@@ -749,7 +749,9 @@ __global__ __launch_bounds__(1024) void k_plan_fill(const uint64_t* __restrict__
 // largest), then one item per regular sub-bucket.  Each item builds the
 // smaller side's chunk into an LDS table and streams the other side's slice
 // past it (probe_item, bj_tiles.h).
-template <int BLOCK, int CAP, int PITEMS>
+// COLLECT (the reference's COLLECT / --fused_probe_breakdown): thread 0 of
+// every workgroup adds its phase times (PT_*) to timers[blockIdx.x * PT_N + p].
+template <int BLOCK, int CAP, int PITEMS, bool COLLECT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_probe(
     const uint32_t* __restrict__ rk, const uint32_t* __restrict__ rv,
     const uint64_t* __restrict__ r_start, const uint32_t* __restrict__ sk,
@@ -757,10 +759,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     const uint32_t* __restrict__ s_cnt, int bits, uint32_t slice,
     const HeavyItem* __restrict__ items, const uint64_t* __restrict__ n_items_p, uint64_t item_cap,
     uint64_t* __restrict__ work_ctr, uint64_t* __restrict__ out_count, uint64_t out_cap,
-    uint32_t* __restrict__ o_key, uint32_t* __restrict__ o_rval, uint32_t* __restrict__ o_sval)
+    uint32_t* __restrict__ o_key, uint32_t* __restrict__ o_rval, uint32_t* __restrict__ o_sval,
+    uint64_t* __restrict__ timers)
 {
     __shared__ ProbeLds<BLOCK, CAP> L;
     __shared__ uint64_t s_item, s_next;
+    uint64_t tacc[PT_N] = {}, tmark = 0;
+    if (COLLECT && threadIdx.x == 0) tmark = phase_clock();
 
     const int nbins = 1 << bits;
     const uint64_t n_heavy = min(*n_items_p, item_cap);
@@ -796,13 +801,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             const uint32_t pn = (uint32_t)min(per, np - p_lo);
             const uint64_t brow0 = (build_r ? r0 : s0) + b_lo;
             const uint64_t prow0 = (build_r ? s0 : r0) + p_lo;
-            probe_item<BLOCK, CAP, PITEMS>(L, (build_r ? rk : sk) + brow0, (build_r ? rv : sv) + brow0, bn,
-                                           (build_r ? sk : rk) + prow0, (build_r ? sv : rv) + prow0, pn,
-                                           build_r, bits, out_count, out_cap, o_key, o_rval, o_sval);
+            phase_tick<COLLECT>(tacc, &tmark, PT_ITEM);
+            probe_item<BLOCK, CAP, PITEMS, COLLECT>(L, (build_r ? rk : sk) + brow0, (build_r ? rv : sv) + brow0,
+                                                    bn, (build_r ? sk : rk) + prow0, (build_r ? sv : rv) + prow0,
+                                                    pn, build_r, bits, out_count, out_cap, o_key, o_rval, o_sval,
+                                                    tacc, &tmark);
         }
         __syncthreads();  // the table is rebuilt by the next item; s_next is written
         item = s_next;
         __syncthreads();
+    }
+    if (COLLECT && threadIdx.x == 0) {
+        phase_tick<COLLECT>(tacc, &tmark, PT_ITEM);
+        for (int p = 0; p < PT_N; p++) timers[(size_t)blockIdx.x * PT_N + p] += tacc[p];
     }
 }
 
@@ -1033,25 +1044,34 @@ hipError_t launch_plan(const uint64_t* r_start, const uint64_t* s_start, const u
     return hipGetLastError();
 }
 
+int probe_grid(int tile, int num_cu) { return tile == 8192 ? num_cu * 2 : num_cu * 4; }
+
+static_assert(PT_N == PROBE_TIMER_SLOTS, "probe timer slots");
+
 hipError_t launch_probe(int tile, const uint32_t* rk, const uint32_t* rv, const uint64_t* r_start,
                         const uint32_t* sk, const uint32_t* sv, const uint64_t* s_start,
                         const uint32_t* s_cnt, int bits, uint32_t slice, const HeavyItem* items,
                         const uint64_t* n_items, uint64_t item_cap, uint64_t* work_ctr, uint64_t* out_count,
                         uint64_t out_cap, uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, int num_cu,
-                        hipStream_t st)
+                        uint64_t* timers, hipStream_t st)
 {
-    if (tile == 8192) {
-        k_probe<512, 8192, 8><<<num_cu * 2, 512, 0, st>>>(rk, rv, r_start, sk, sv, s_start, s_cnt, bits,
-                                                          slice, items, n_items, item_cap, work_ctr,
-                                                          out_count, out_cap, o_key, o_rval, o_sval);
-    } else if (tile == 4096) {
-        k_probe<256, 4096, 8><<<num_cu * 4, 256, 0, st>>>(rk, rv, r_start, sk, sv, s_start, s_cnt, bits,
-                                                          slice, items, n_items, item_cap, work_ctr,
-                                                          out_count, out_cap, o_key, o_rval, o_sval);
-    } else {
-        return hipErrorInvalidValue;
-    }
+#define BJ_PROBE(B, C, G)                                                                                  \
+    do {                                                                                                 \
+        if (timers)                                                                                      \
+            k_probe<B, C, 8, true><<<(G), B, 0, st>>>(rk, rv, r_start, sk, sv, s_start, s_cnt, bits, slice, \
+                                                      items, n_items, item_cap, work_ctr, out_count,     \
+                                                      out_cap, o_key, o_rval, o_sval, timers);           \
+        else                                                                                             \
+            k_probe<B, C, 8, false><<<(G), B, 0, st>>>(rk, rv, r_start, sk, sv, s_start, s_cnt, bits,    \
+                                                       slice, items, n_items, item_cap, work_ctr,        \
+                                                       out_count, out_cap, o_key, o_rval, o_sval, nullptr); \
+    } while (0)
+    if (tile == 8192) BJ_PROBE(512, 8192, probe_grid(8192, num_cu));
+    else if (tile == 4096) BJ_PROBE(256, 4096, probe_grid(4096, num_cu));
+    else return hipErrorInvalidValue;
+#undef BJ_PROBE
     return hipGetLastError();
 }
+
 
 }  // namespace bj

@@ -238,6 +238,7 @@ int bj_gpu_main(const bj_main_args* a, bj_result* res)
     p.num_hashes = a->override_num_hashes;
     p.answer_scale = a->answer_scale;
     p.s_chunks = a->s_chunks;
+    p.probe_breakdown = a->fused_probe_breakdown ? 1 : 0;
     p.inputs_on_device = 0;
     p.collect_phase_times = 1;
 
@@ -290,6 +291,11 @@ int bj_gpu_main(const bj_main_args* a, bj_result* res)
         std::printf("[2-2] Probe           %10.3f\n", res->phase_ms[BJ_PH_PROBE]);
         std::printf("Total (excl. H2D)     %10.3f\n", res->phase_ms[BJ_PH_TOTAL]);
         if (a->fused_probe_breakdown) {
+            std::printf("      [2-2a] Work items    %10.3f\n", res->probe_phase_ms[BJ_PP_ITEMS]);
+            std::printf("      [2-2a2] Table clear  %10.3f\n", res->probe_phase_ms[BJ_PP_CLEAR]);
+            std::printf("      [2-2b] Build         %10.3f\n", res->probe_phase_ms[BJ_PP_BUILD]);
+            std::printf("      [2-2c] Probe count   %10.3f\n", res->probe_phase_ms[BJ_PP_COUNT]);
+            std::printf("      [2-2d] Rows          %10.3f\n", res->probe_phase_ms[BJ_PP_WRITE]);
             const double t = res->phase_ms[BJ_PH_TOTAL] * 1e-3;
             std::printf("input tuples/s (excl. H2D): %.4e\n",
                         t > 0 ? (double)(a->r_size + a->s_size) / t : 0.0);

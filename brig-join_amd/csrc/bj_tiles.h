@@ -387,6 +387,25 @@ struct alignas(16) ProbeLds {
     unsigned long long s_base;
 };
 
+// Probe phase timers (one accumulator per workgroup, kept by thread 0, in
+// ticks of the constant-rate wall clock): fetching work items, clearing the
+// table, building it, counting the rounds' matches (through the reservation),
+// writing the rounds' rows.
+enum { PT_ITEM = 0, PT_CLEAR, PT_BUILD, PT_COUNT, PT_WRITE, PT_N };
+
+__device__ __forceinline__ uint64_t phase_clock() { return wall_clock64(); }
+
+// thread 0 of a collecting workgroup: acc[ph] += now - mark, mark = now
+template <bool COLLECT>
+__device__ __forceinline__ void phase_tick(uint64_t* acc, uint64_t* mark, int ph)
+{
+    if (COLLECT && threadIdx.x == 0) {
+        const uint64_t t = phase_clock();
+        acc[ph] += t - *mark;
+        *mark = t;
+    }
+}
+
 // 64-bit value that is the same in every lane, in scalar registers
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v)
 {
@@ -395,14 +414,17 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <int BLOCK, int CAP, int PITEMS>
+// COLLECT: thread 0 adds the item's phase times to tacc[PT_CLEAR..PT_WRITE]
+// (tmark: the time the item started; left at the item's end).
+template <int BLOCK, int CAP, int PITEMS, bool COLLECT = false>
 __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32_t* __restrict__ bkp,
                                            const uint32_t* __restrict__ bvp, int bn,
                                            const uint32_t* __restrict__ pkp, const uint32_t* __restrict__ pvp,
                                            uint32_t pn, bool build_r, int bits,
                                            uint64_t* __restrict__ out_count, uint64_t out_cap,
                                            uint32_t* __restrict__ o_key, uint32_t* __restrict__ o_rval,
-                                           uint32_t* __restrict__ o_sval)
+                                           uint32_t* __restrict__ o_sval, uint64_t* tacc = nullptr,
+                                           uint64_t* tmark = nullptr)
 {
     constexpr int ROUND = BLOCK * PITEMS;
     constexpr int BPT = CAP / BLOCK;  // build tuples per thread
@@ -431,6 +453,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         cnt[threadIdx.x] = 0u;
     }
     __syncthreads();
+    phase_tick<COLLECT>(tacc, tmark, PT_CLEAR);
     // build: rank each tuple in its bucket, scan the counts, place;
     // bucket (< 2^15) and rank (< 2^16) of each build tuple, packed
     uint32_t bkey[BPT], btr[BPT];
@@ -491,6 +514,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         }
     }
     __syncthreads();
+    phase_tick<COLLECT>(tacc, tmark, PT_BUILD);
 
     // key loads clamped to the last key (pn >= 1): no branches around loads
     uint32_t nk[PITEMS];
@@ -545,6 +569,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         }
         uint32_t total;
         const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, L.wtmp, &total);
+        phase_tick<COLLECT>(tacc, tmark, PT_COUNT);
         if (total == 0) {  // uniform
             prefetch();
             continue;
@@ -603,6 +628,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
                 orv[j] = L.stage[1][j];
                 osv[j] = L.stage[2][j];
             }
+            phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
             continue;
         }
         __syncthreads();
@@ -633,6 +659,7 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
                 }
             }
         }
+        phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
     }
     // the caller synchronises before the next item rebuilds the table
 }

@@ -58,6 +58,19 @@ enum {
     BJ_NUM_PHASES
 };
 
+/* ---- phases inside the probe kernel (bj_params.probe_breakdown; the
+ * reference's COLLECT timers, src/kernel_functions.cu:300-302, 442-451, turned
+ * into times as src/join_executor.cu:56-98 does: each phase's share of all
+ * workgroups' clock ticks times the probe kernel's time) ------------------ */
+enum {
+    BJ_PP_ITEMS = 0,     /* fetching work items and their sizes (preamble)           */
+    BJ_PP_CLEAR,         /* clearing the LDS table's bucket counts (memset)          */
+    BJ_PP_BUILD,         /* ranking, scanning and placing the build side (build)     */
+    BJ_PP_COUNT,         /* probe rounds: buckets, match counts, scan, reservation   */
+    BJ_PP_WRITE,         /* probe rounds: value loads and row writes                 */
+    BJ_NUM_PROBE_PHASES
+};
+
 /* ---- per-kernel timing slots (milliseconds, hipEvent on the join stream) */
 enum {
     BJ_K_HIST_R = 0,     /* k_hist over R (joint radix histogram)                 */
@@ -121,6 +134,9 @@ typedef struct bj_params {
                                        bits for each first-pass bucket's filter slice
                                        to fit an XCD's L2, else pass1_bits)            */
     int32_t scatter_wg;             /* scatter workgroup size, 512 or 1024; 0 = auto   */
+    int32_t probe_breakdown;        /* --fused_probe_breakdown: 1 = time the probe's
+                                       phases (bj_result.probe_phase_ms; implies
+                                       collect_phase_times)                          */
 } bj_params;
 
 /* ---- join result ------------------------------------------------------- */
@@ -146,6 +162,7 @@ typedef struct bj_result {
     int32_t  pass1_bits;      /* radix bits of the first physical scatter pass       */
     int32_t  s_pass1_bits;    /* the same for S                                      */
     int32_t  s_chunks;        /* chunks S was joined in (1 = one pass)               */
+    double   probe_phase_ms[BJ_NUM_PROBE_PHASES];  /* with probe_breakdown, else 0   */
 } bj_result;
 
 /* ---- the plan a join would use (no device needed) ---------------------- */
