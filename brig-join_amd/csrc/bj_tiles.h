@@ -378,13 +378,15 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
 template <int BLOCK, int CAP>
 struct alignas(16) ProbeLds {
     static constexpr int STAGE = 2 * BLOCK;  // rows of one round staged in LDS
-    uint32_t stage[3][STAGE];  // key, R value, S value of the round's rows (first:
+    uint32_t stage_p[STAGE];   // probe-side row of each of the round's rows (first:
                                // short LDS offsets for the per-row writes)
+    uint16_t stage_b[STAGE];   // build-side row (in the chunk)
     uint32_t hkey[CAP];        // keys by bucket (bucket counts while the table is built)
     uint16_t hidx[CAP];
     uint16_t bst[CAP + 2];     // bucket starts, CAP buckets at most
     uint32_t wtmp[BLOCK / WAVE + 1];
-    unsigned long long s_base;
+    unsigned long long s_base;    // reservation of the staged round being stored
+    unsigned long long s_direct;  // reservation of a round written from the lanes
 };
 
 // Probe phase timers (one accumulator per workgroup, kept by thread 0, in
@@ -520,23 +522,66 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
     uint32_t nk[PITEMS];
 #pragma unroll
     for (int it = 0; it < PITEMS; it++) nk[it] = pkp[min((uint32_t)it * BLOCK + threadIdx.x, pn - 1u)];
+    // Rows are written one round late: a round stages the (probe row, build row)
+    // pairs of its matches in LDS, reserves their places (one atomic by thread
+    // 0, its result read a round later), and each thread loads the key and
+    // values of rows tid and tid + BLOCK; the next round stores them after its
+    // own count.  No wait for a reservation or a value load is on a round's path.
+    uint32_t wk0, wp0, wb0, wk1, wp1, wb1;  // this thread's pending rows: key, probe / build value
+    uint32_t ptotal = 0;            // rows of the pending round (uniform)
+    unsigned long long psb = 0;     // thread 0: the pending round's reservation
+    // The six loads end every round, also where nothing is staged (rows 0 of
+    // both sides, unused): the loads in flight at the top of a round are then
+    // the same on every path (the next keys, then these six), so waiting for the
+    // keys never waits for these.
+    auto load_rows = [&](uint32_t total) {
+        uint32_t p0 = 0, b0 = 0, p1 = 0, b1 = 0;
+        if (total) {  // uniform; rows tid and tid + BLOCK, clamped to the last row
+            const uint32_t j0 = min((uint32_t)threadIdx.x, total - 1u);
+            const uint32_t j1 = min((uint32_t)threadIdx.x + BLOCK, total - 1u);
+            p0 = L.stage_p[j0];
+            b0 = L.stage_b[j0];
+            p1 = L.stage_p[j1];
+            b1 = L.stage_b[j1];
+        }
+        // per-lane rows: vector loads on every path (a known row 0 would become
+        // a scalar load, which the vector load counter does not see)
+        asm volatile("" : "+v"(p0), "+v"(b0), "+v"(p1), "+v"(b1));
+        wk0 = pkp[p0];
+        wk1 = pkp[p1];
+        wp0 = pvp[p0];
+        wp1 = pvp[p1];
+        wb0 = bvp[b0];
+        wb1 = bvp[b1];
+    };
+    auto flush = [&]() {            // after a barrier that published L.s_base
+        const uint64_t sb = uniform_u64(L.s_base);
+        const uint32_t nw = sb >= out_cap ? 0u : (uint32_t)min<uint64_t>(ptotal, out_cap - sb);
+        const uint32_t j0 = threadIdx.x, j1 = threadIdx.x + BLOCK;
+        if (j0 < nw) {
+            o_key[sb + j0] = wk0;
+            o_rval[sb + j0] = build_r ? wb0 : wp0;
+            o_sval[sb + j0] = build_r ? wp0 : wb0;
+        }
+        if (j1 < nw) {
+            o_key[sb + j1] = wk1;
+            o_rval[sb + j1] = build_r ? wb1 : wp1;
+            o_sval[sb + j1] = build_r ? wp1 : wb1;
+        }
+        ptotal = 0;
+    };
+    load_rows(0);
     for (uint32_t base = 0; base < pn; base += ROUND) {
         uint32_t kk[PITEMS], qs[PITEMS], qe[PITEMS];
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
-        // the next round's keys are issued after this round's value loads, so the
-        // wait for those values does not cover them
-        auto prefetch = [&]() {
-            const uint32_t nb0 = base + ROUND;
-            if (nb0 + ROUND <= pn) {
+        // the next round's keys, issued before this round's value loads so that
+        // the next round's wait for them does not cover those; clamped, and
+        // issued in the last round too, so that every round has the same loads
+        // in flight (the compiler's waits then count past them)
 #pragma unroll
-                for (int it = 0; it < PITEMS; it++) nk[it] = pkp[nb0 + (uint32_t)it * BLOCK + threadIdx.x];
-            } else if (nb0 < pn) {
-#pragma unroll
-                for (int it = 0; it < PITEMS; it++)
-                    nk[it] = pkp[min(nb0 + (uint32_t)it * BLOCK + threadIdx.x, pn - 1u)];
-            }
-        };
+        for (int it = 0; it < PITEMS; it++)
+            nk[it] = pkp[min(base + ROUND + (uint32_t)it * BLOCK + threadIdx.x, pn - 1u)];
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
             // two 16-bit reads: one 32-bit read at a 2-byte aligned address is
@@ -567,22 +612,52 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
             hitmask |= (m ? 1u : 0u) << it;
             multi |= (m > 1 ? 1u : 0u) << it;
         }
+        // the pending round's reservation (issued a round ago; the staging
+        // barrier of that round separates this write from its readers)
+        if (threadIdx.x == 0 && ptotal) L.s_base = psb;
         uint32_t total;
         const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, L.wtmp, &total);
         phase_tick<COLLECT>(tacc, tmark, PT_COUNT);
+        if (ptotal) {
+            flush();
+            if (total == 0) phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
+        }
         if (total == 0) {  // uniform
-            prefetch();
+            load_rows(0);
             continue;
         }
+        if (total <= (uint32_t)STAGE) {  // uniform
+            if (threadIdx.x == 0) psb = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
+            uint32_t w = pre;
+#pragma unroll
+            for (int it = 0; it < PITEMS; it++) {
+                if (hitmask & (1u << it)) {
+                    const uint32_t prow = base + (uint32_t)it * BLOCK + threadIdx.x;
+                    L.stage_p[w] = prow;
+                    L.stage_b[w] = L.hidx[q0[it]];
+                    w++;
+                    if (multi & (1u << it)) {
+                        for (uint32_t q = q0[it] + 1; q < qe[it]; q++) {
+                            if (L.hkey[q] == kk[it]) {
+                                L.stage_p[w] = prow;
+                                L.stage_b[w] = L.hidx[q];
+                                w++;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            load_rows(total);
+            ptotal = total;
+            phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
+            continue;
+        }
+        // a round with more rows than the stage (duplicate-heavy keys): rows
+        // written from the lanes, after the reservation
         if (threadIdx.x == 0)
-            L.s_base = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
-        // while the reservation is in flight: the values of both sides of every
-        // first match, all loads of the round issued together
+            L.s_direct = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
         uint32_t pv[PITEMS], bv[PITEMS], hx[PITEMS];
-        // The loads are unconditional, lanes without a hit read element 0 (one
-        // line per wave): loads under a branch would make the stores below wait
-        // for every load in flight, the next round's keys included.  The build
-        // rows are read from LDS first, all together (q0 = 0 without a hit).
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) hx[it] = L.hidx[q0[it]];
 #pragma unroll
@@ -591,48 +666,8 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
             pv[it] = pvp[hit ? base + (uint32_t)it * BLOCK + threadIdx.x : 0u];
             bv[it] = bvp[hx[it]];
         }
-        prefetch();
-        if (total <= (uint32_t)STAGE) {  // uniform
-            // rows into LDS at their round offsets, then stored by consecutive
-            // threads; the stage is rewritten only after the next round's scan
-            uint32_t w = pre;
-#pragma unroll
-            for (int it = 0; it < PITEMS; it++) {
-                if (hitmask & (1u << it)) {
-                    const uint32_t pval = pv[it];
-                    L.stage[0][w] = kk[it];
-                    L.stage[1][w] = build_r ? bv[it] : pval;
-                    L.stage[2][w] = build_r ? pval : bv[it];
-                    w++;
-                    if (multi & (1u << it)) {
-                        for (uint32_t q = q0[it] + 1; q < qe[it]; q++) {
-                            if (L.hkey[q] == kk[it]) {
-                                const uint32_t bval = bvp[L.hidx[q]];
-                                L.stage[0][w] = kk[it];
-                                L.stage[1][w] = build_r ? bval : pval;
-                                L.stage[2][w] = build_r ? pval : bval;
-                                w++;
-                            }
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            const uint64_t sb = uniform_u64(L.s_base);
-            const uint32_t nw = sb >= out_cap ? 0u : (uint32_t)min<uint64_t>(total, out_cap - sb);
-            uint32_t* ok = o_key + sb;
-            uint32_t* orv = o_rval + sb;
-            uint32_t* osv = o_sval + sb;
-            for (uint32_t j = threadIdx.x; j < nw; j += BLOCK) {
-                ok[j] = L.stage[0][j];
-                orv[j] = L.stage[1][j];
-                osv[j] = L.stage[2][j];
-            }
-            phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
-            continue;
-        }
         __syncthreads();
-        uint64_t w = L.s_base + pre;
+        uint64_t w = L.s_direct + pre;
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) {
             if (hitmask & (1u << it)) {
@@ -659,6 +694,15 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
                 }
             }
         }
+        // L.s_direct is rewritten only after the next round's scan barriers
+        load_rows(0);
+        phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
+    }
+    // the last staged round
+    if (ptotal) {
+        if (threadIdx.x == 0) L.s_base = psb;
+        __syncthreads();
+        flush();
         phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
     }
     // the caller synchronises before the next item rebuilds the table
