@@ -369,17 +369,19 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
 // short contiguous run of keys: no pointer chasing.
 // The probe side (pn keys) streams through in rounds of BLOCK*PITEMS keys (the
 // next round is loaded while the current one is processed).  Each round counts
-// its matches, reserves its rows with one global atomic after a block scan,
-// then writes them: staged in LDS and stored by consecutive threads when the
-// round has at most STAGE rows, else directly from the lanes.  Both values of a
-// row are read only for matches: the build side's from its (cache-resident)
-// chunk, the probe side's from its slice.  Rows past out_cap are counted, not
-// written.
+// its matches and block-scans the counts.  A round with at most STAGE rows
+// stages them in LDS as (probe row, build row) pairs, reserves them with one
+// global atomic and loads their keys and values, one or two rows per thread;
+// the next round stores them after its own count (stores by consecutive
+// threads).  Larger rounds reserve and write from the lanes at once.  Both
+// values of a row are read only for matches: the build side's from its
+// (cache-resident) chunk, the probe side's from its slice.  Rows past out_cap
+// are counted, not written.
 template <int BLOCK, int CAP>
 struct alignas(16) ProbeLds {
     static constexpr int STAGE = 2 * BLOCK;  // rows of one round staged in LDS
     uint32_t stage_p[STAGE];   // probe-side row of each of the round's rows (first:
-                               // short LDS offsets for the per-row writes)
+                               // short LDS offsets for the per-row accesses)
     uint16_t stage_b[STAGE];   // build-side row (in the chunk)
     uint32_t hkey[CAP];        // keys by bucket (bucket counts while the table is built)
     uint16_t hidx[CAP];
