@@ -124,7 +124,9 @@ typedef struct bj_params {
                                        partitioned (the reference's order); 2 = after
                                        the first-level pass, one filter slice per
                                        bucket resident in an XCD's L2; 0 = auto (1 when
-                                       the filter fits the 256 MB MALL, else 2)        */
+                                       the whole filter fits an XCD's L2, 3.5 MB; else
+                                       2: a filter probed at random from the MALL or
+                                       HBM is several times slower)                    */
     int32_t pass1_bits;             /* radix bits of the first physical scatter pass
                                        (the rest go to the second); -1 = auto (half).
                                        The resulting sub-bucket layout does not depend
