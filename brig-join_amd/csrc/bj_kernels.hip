@@ -194,30 +194,52 @@ __global__ __launch_bounds__(1024) void k_colsums(const uint32_t* __restrict__ p
 }
 
 // k_starts: one workgroup scans the bin totals into the sub-bucket starts
-// (sb_start[nbins] = number of tuples); thread t owns a run of at most 32 bins.
+// (sb_start[nbins] = number of tuples).  nbins is a power of two up to 2^15:
+// thread t owns bins [t * per, (t + 1) * per), per = nbins / 1024 (at least 1),
+// read as 16-byte vectors and written as pairs of starts from 4 bins on.
 __global__ __launch_bounds__(1024) void k_starts(const uint32_t* __restrict__ totals, int nbins,
                                                  uint64_t* __restrict__ sb_start)
 {
     constexpr int MAXPER = 32;
     __shared__ uint64_t wtmp[1024 / WAVE + 1];
-    const int per = (nbins + 1023) / 1024;
-    const int b0 = min((int)threadIdx.x * per, nbins), b1 = min(b0 + per, nbins);
+    const int per = max(1, nbins >> 10);
+    const int b0 = (int)threadIdx.x * per;
     uint32_t v[MAXPER];
-    uint64_t sum = 0;
-    // clamped loads, all in flight together, masked afterwards
+    if (per >= 4) {
+        const uint4* t4 = (const uint4*)(totals + b0);
 #pragma unroll
-    for (int i = 0; i < MAXPER; i++) v[i] = totals[min(b0 + i, nbins - 1)];
+        for (int j = 0; j < MAXPER / 4; j++) {
+            const uint4 q = (4 * j < per) ? t4[j] : make_uint4(0u, 0u, 0u, 0u);
+            v[4 * j] = q.x;
+            v[4 * j + 1] = q.y;
+            v[4 * j + 2] = q.z;
+            v[4 * j + 3] = q.w;
+        }
+    } else {
 #pragma unroll
-    for (int i = 0; i < MAXPER; i++) {
-        v[i] = (b0 + i < b1) ? v[i] : 0u;
-        sum += v[i];
+        for (int i = 0; i < 4; i++) v[i] = (i < per && b0 + i < nbins) ? totals[b0 + i] : 0u;
+#pragma unroll
+        for (int i = 4; i < MAXPER; i++) v[i] = 0u;
     }
+    uint64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < MAXPER; i++) sum += v[i];
     uint64_t tot;
     uint64_t ex = block_exclusive_scan<1024, uint64_t>(sum, wtmp, &tot);
+    if (per >= 4) {
+        ulonglong2* o2 = (ulonglong2*)(sb_start + b0);
 #pragma unroll
-    for (int i = 0; i < MAXPER; i++) {
-        if (b0 + i < b1) sb_start[b0 + i] = ex;
-        ex += v[i];
+        for (int j = 0; j < MAXPER / 2; j++) {
+            const uint64_t e0 = ex, e1 = ex + v[2 * j];
+            ex = e1 + v[2 * j + 1];
+            if (2 * j < per) o2[j] = make_ulonglong2(e0, e1);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (i < per && b0 + i < nbins) sb_start[b0 + i] = ex;
+            ex += v[i];
+        }
     }
     if (threadIdx.x == 0) sb_start[nbins] = tot;
 }
