@@ -717,29 +717,52 @@ __global__ __launch_bounds__(1024) void k_plan_fill(const uint64_t* __restrict__
                                                     uint64_t* __restrict__ item_start, uint64_t item_cap,
                                                     HeavyItem* __restrict__ items, uint64_t* __restrict__ err)
 {
+    // nbins is a power of two up to 2^15: thread t owns sub-buckets
+    // [t * per, (t + 1) * per), per = nbins / 1024 (at least 1); counts read as
+    // 16-byte vectors and starts written in pairs from 4 sub-buckets on
     constexpr int MAXPER = 32;
     __shared__ uint64_t wtmp[1024 / WAVE + 1];
-    const int per = (nbins + 1023) / 1024;
-    const int b0 = min((int)threadIdx.x * per, nbins), b1 = min(b0 + per, nbins);
+    const int per = max(1, nbins >> 10);
+    const int b0 = (int)threadIdx.x * per, b1 = min(b0 + per, nbins);
     uint32_t v[MAXPER];
-    uint64_t sum = 0;
-    // clamped loads, all in flight together, masked afterwards
+    if (per >= 4) {
+        const uint4* n4 = (const uint4*)(n_items + b0);
 #pragma unroll
-    for (int i = 0; i < MAXPER; i++) v[i] = n_items[min(b0 + i, nbins - 1)];
+        for (int j = 0; j < MAXPER / 4; j++) {
+            const uint4 q = (4 * j < per) ? n4[j] : make_uint4(0u, 0u, 0u, 0u);
+            v[4 * j] = q.x;
+            v[4 * j + 1] = q.y;
+            v[4 * j + 2] = q.z;
+            v[4 * j + 3] = q.w;
+        }
+    } else {
 #pragma unroll
-    for (int i = 0; i < MAXPER; i++) {
-        v[i] = (b0 + i < b1) ? v[i] : 0u;
-        sum += v[i];
+        for (int i = 0; i < 4; i++) v[i] = (b0 + i < b1) ? n_items[b0 + i] : 0u;
+#pragma unroll
+        for (int i = 4; i < MAXPER; i++) v[i] = 0u;
     }
+    uint64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < MAXPER; i++) sum += v[i];
     uint64_t total;
     uint64_t base = block_exclusive_scan<1024, uint64_t>(sum, wtmp, &total);
     if (threadIdx.x == 0) item_start[nbins] = total;
+    if (per >= 4) {
+        ulonglong2* o2 = (ulonglong2*)(item_start + b0);
+        uint64_t e = base;
+#pragma unroll
+        for (int j = 0; j < MAXPER / 2; j++) {
+            const uint64_t e0 = e, e1 = e + v[2 * j];
+            e = e1 + v[2 * j + 1];
+            if (2 * j < per) o2[j] = make_ulonglong2(e0, e1);
+        }
+    }
     bool over = false;
 #pragma unroll
     for (int i = 0; i < MAXPER; i++) {
         const int sb = b0 + i;
         if (sb < b1) {
-            item_start[sb] = base;
+            if (per < 4) item_start[sb] = base;
             if (v[i]) {
                 uint64_t r0, nr, s0, ns;
                 sb_sizes(r_start, s_start, s_cnt, sb, r0, nr, s0, ns);
