@@ -170,20 +170,26 @@ def run_batch(args, n_joins: int) -> dict:
     ms_step = wall / args.steps * 1e3
     value = (tot_r + tot_s) / (wall / args.steps)
 
-    # the single-join path, one join at a time (same inputs, its own automatic plan)
+    # the single-join path, one join at a time (same inputs, its own automatic
+    # plan): timed without events between kernels, then once more with them for
+    # its device time
+    for p, _ in plain:
+        p.collect_phase_times = 0
     for p, d in plain[:4]:
         ctx.join(p, *d)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain_dev = 0.0
     for i, (p, d) in enumerate(plain):
         r = ctx.join(p, *d)
-        plain_dev += r.phase_ms[6]
         if r.match_count != expect[i]:
             raise RuntimeError(f"single-join count differs at join {i}")
     torch.cuda.synchronize()
     plain_wall = time.perf_counter() - t0
     plain_value = (tot_r + tot_s) / plain_wall
+    plain_dev = 0.0
+    for p, d in plain:
+        p.collect_phase_times = 1
+        plain_dev += ctx.join(p, *d).phase_ms[6]
 
     # whole-batch algorithmic bytes (per join: the two-pass formula of its configuration)
     alg = 0.0
@@ -249,7 +255,7 @@ def run_serve(args, opts: dict) -> dict:
     for ri, sk, sv, ns, m in pool:
         rk, rv, nr = R[ri]
         p = bj.default_params(r_size=nr, s_size=ns, inputs_on_device=1, use_bf=1, bf_bits_per_key=10.0,
-                              num_hashes=3)
+                              num_hashes=3, collect_phase_times=0)
         ctx.join(p, rk, rv, sk, sv)  # warm
         t0 = time.perf_counter()
         res = ctx.join(p, rk, rv, sk, sv)
@@ -446,13 +452,25 @@ def main() -> int:
     p = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=0 if args.host_inputs else 1, **params)
     bloom = bool(params.get("use_bf"))
 
+    # The timed steps record no events between kernels (each costs the device
+    # a few microseconds); the per-kernel times and the roofline come from the
+    # same number of instrumented joins after them.
+    p.collect_phase_times = 0
     for _ in range(args.warmup):
         res = ctx.join(p, rk, rv, sk, sv)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = ctx.join(p, rk, rv, sk, sv)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if res.match_count != m_s:
+        print(f"wrong match count {res.match_count} != {m_s}", file=sys.stderr)
+        return 1
+    p.collect_phase_times = 1
     kern_sum = {k: 0.0 for k in bj.KERNELS}
     total_dev = 0.0
     h2d_sum = 0.0
-    t0 = time.perf_counter()
     for _ in range(args.steps):
         res = ctx.join(p, rk, rv, sk, sv)
         for k, v in res.kernels().items():
@@ -460,7 +478,6 @@ def main() -> int:
         total_dev += res.phase_ms[6]
         h2d_sum += res.phase_ms[0]
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     if res.match_count != m_s:
         print(f"wrong match count {res.match_count} != {m_s}", file=sys.stderr)
         return 1
