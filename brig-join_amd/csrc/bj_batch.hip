@@ -378,6 +378,9 @@ __global__ void kb_plan_fill(BatchDev B, uint32_t nsb, const uint32_t* __restric
 // Persistent over the sub-buckets [sb_lo, sb_hi) of all joins of one tile
 // class: first their vlsplit items, then one item per regular sub-bucket.
 // Rows of join b go to its output region, reserved on its own counter.
+// Probe keys per thread per round (as the single join's probe).
+constexpr int KB_PROBE_ITEMS = 10;
+
 template <int BLOCK, int CAP, int PITEMS>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_probe(
     BatchDev B, uint32_t sb_lo, uint32_t sb_hi, const uint32_t* __restrict__ rk, const uint32_t* __restrict__ rv,
@@ -569,11 +572,11 @@ hipError_t launch_batch_probe(const BatchDev& B, int tile, uint32_t sb_lo, uint3
 {
     if (sb_hi <= sb_lo) return hipSuccess;
     if (tile == 8192) {
-        kb_probe<512, 8192, 8><<<num_cu * 2, 512, 0, st>>>(B, sb_lo, sb_hi, rk, rv, r_start, r_cnt, sk, sv,
+        kb_probe<512, 8192, KB_PROBE_ITEMS><<<num_cu * 2, 512, 0, st>>>(B, sb_lo, sb_hi, rk, rv, r_start, r_cnt, sk, sv,
                                                            s_start, s_cnt, slice, items, item_start, item_cap,
                                                            work_ctr, jc, o_key, o_rval, o_sval);
     } else if (tile == 4096) {
-        kb_probe<256, 4096, 8><<<num_cu * 4, 256, 0, st>>>(B, sb_lo, sb_hi, rk, rv, r_start, r_cnt, sk, sv,
+        kb_probe<256, 4096, KB_PROBE_ITEMS><<<num_cu * 4, 256, 0, st>>>(B, sb_lo, sb_hi, rk, rv, r_start, r_cnt, sk, sv,
                                                            s_start, s_cnt, slice, items, item_start, item_cap,
                                                            work_ctr, jc, o_key, o_rval, o_sval);
     } else {
