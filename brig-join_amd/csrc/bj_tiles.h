@@ -380,14 +380,14 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
 template <int BLOCK, int CAP>
 struct alignas(16) ProbeLds {
     static constexpr int STAGE = 2 * BLOCK;  // rows of one round staged in LDS
-    uint32_t stage_p[STAGE];   // probe-side row of each of the round's rows (first:
-                               // short LDS offsets for the per-row accesses)
-    uint16_t stage_b[STAGE];   // build-side row (in the chunk)
+    uint32_t stage_p[2][STAGE];  // probe-side row of each of a round's rows, two
+                                 // rounds (first: short LDS offsets)
+    uint16_t stage_b[2][STAGE];  // build-side row (in the chunk)
     uint32_t hkey[CAP];        // keys by bucket (bucket counts while the table is built)
     uint16_t hidx[CAP];
     uint16_t bst[CAP + 2];     // bucket starts, CAP buckets at most
     uint32_t wtmp[BLOCK / WAVE + 1];
-    unsigned long long s_base;    // reservation of the staged round being stored
+    unsigned long long s_base[2];  // reservations of staged rounds being stored
     unsigned long long s_direct;  // reservation of a round written from the lanes
 };
 
@@ -524,27 +524,32 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
     uint32_t nk[PITEMS];
 #pragma unroll
     for (int it = 0; it < PITEMS; it++) nk[it] = pkp[min((uint32_t)it * BLOCK + threadIdx.x, pn - 1u)];
-    // Rows are written one round late: a round stages the (probe row, build row)
-    // pairs of its matches in LDS, reserves their places (one atomic by thread
-    // 0, its result read a round later), and each thread loads the key and
-    // values of rows tid and tid + BLOCK; the next round stores them after its
-    // own count.  No wait for a reservation or a value load is on a round's path.
-    uint32_t wk0, wp0, wb0, wk1, wp1, wb1;  // this thread's pending rows: key, probe / build value
-    uint32_t ptotal = 0;            // rows of the pending round (uniform)
-    unsigned long long psb = 0;     // thread 0: the pending round's reservation
-    // The six loads end every round, also where nothing is staged (rows 0 of
-    // both sides, unused): the loads in flight at the top of a round are then
-    // the same on every path (the next keys, then these six), so waiting for the
-    // keys never waits for these.
-    auto load_rows = [&](uint32_t total) {
+    // Rows are written two rounds late, with two barriers per round (those of
+    // the count scan): round r stages the (probe row, build row) pairs of its
+    // matches in stage[r & 1] and reserves their places (one atomic by thread 0);
+    // round r + 1 loads the key and values of rows tid and tid + BLOCK; round
+    // r + 2 stores them after its count, at the reservation thread 0 published
+    // in s_base[r & 1] before that round's scan.  The scan barriers of the round
+    // in between separate every LDS write from the reads of the same slot.  No
+    // wait for a reservation or a value load is on a round's path.
+    uint32_t wk0, wp0, wb0, wk1, wp1, wb1;  // loaded rows: key, probe / build value
+    uint32_t n_loaded = 0;          // rows in the registers, stored this round (uniform)
+    uint32_t n_staged = 0;          // rows staged last round, loaded this round (uniform)
+    unsigned long long sb_loaded = 0, sb_staged = 0;  // thread 0: their reservations
+    uint32_t par = 0;               // round parity
+    // The six loads are issued in every round, also where nothing is staged
+    // (rows 0 of both sides, unused): the loads in flight at the top of a round
+    // are then the same on every path (the next keys, then these six), so
+    // waiting for the keys never waits for these.
+    auto load_rows = [&](uint32_t n, int buf) {
         uint32_t p0 = 0, b0 = 0, p1 = 0, b1 = 0;
-        if (total) {  // uniform; rows tid and tid + BLOCK, clamped to the last row
-            const uint32_t j0 = min((uint32_t)threadIdx.x, total - 1u);
-            const uint32_t j1 = min((uint32_t)threadIdx.x + BLOCK, total - 1u);
-            p0 = L.stage_p[j0];
-            b0 = L.stage_b[j0];
-            p1 = L.stage_p[j1];
-            b1 = L.stage_b[j1];
+        if (n) {  // uniform; rows tid and tid + BLOCK, clamped to the last row
+            const uint32_t j0 = min((uint32_t)threadIdx.x, n - 1u);
+            const uint32_t j1 = min((uint32_t)threadIdx.x + BLOCK, n - 1u);
+            p0 = L.stage_p[buf][j0];
+            b0 = L.stage_b[buf][j0];
+            p1 = L.stage_p[buf][j1];
+            b1 = L.stage_b[buf][j1];
         }
         // per-lane rows: vector loads on every path (a known row 0 would become
         // a scalar load, which the vector load counter does not see)
@@ -556,9 +561,9 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
         wb0 = bvp[b0];
         wb1 = bvp[b1];
     };
-    auto flush = [&]() {            // after a barrier that published L.s_base
-        const uint64_t sb = uniform_u64(L.s_base);
-        const uint32_t nw = sb >= out_cap ? 0u : (uint32_t)min<uint64_t>(ptotal, out_cap - sb);
+    auto flush = [&](uint32_t n, int slot) {  // after a barrier that published L.s_base[slot]
+        const uint64_t sb = uniform_u64(L.s_base[slot]);
+        const uint32_t nw = sb >= out_cap ? 0u : (uint32_t)min<uint64_t>(n, out_cap - sb);
         const uint32_t j0 = threadIdx.x, j1 = threadIdx.x + BLOCK;
         if (j0 < nw) {
             o_key[sb + j0] = wk0;
@@ -570,10 +575,9 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
             o_rval[sb + j1] = build_r ? wb1 : wp1;
             o_sval[sb + j1] = build_r ? wp1 : wb1;
         }
-        ptotal = 0;
     };
-    load_rows(0);
-    for (uint32_t base = 0; base < pn; base += ROUND) {
+    load_rows(0, 0);
+    for (uint32_t base = 0; base < pn; base += ROUND, par ^= 1u) {
         uint32_t kk[PITEMS], qs[PITEMS], qe[PITEMS];
 #pragma unroll
         for (int it = 0; it < PITEMS; it++) kk[it] = nk[it];
@@ -614,44 +618,46 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
             hitmask |= (m ? 1u : 0u) << it;
             multi |= (m > 1 ? 1u : 0u) << it;
         }
-        // the pending round's reservation (issued a round ago; the staging
-        // barrier of that round separates this write from its readers)
-        if (threadIdx.x == 0 && ptotal) L.s_base = psb;
+        // the reservation of the rows stored this round (staged two rounds ago,
+        // in slot par; its last reader was that round's store, before the last
+        // round's scan barriers)
+        if (threadIdx.x == 0 && n_loaded) L.s_base[par] = sb_loaded;
         uint32_t total;
         const uint32_t pre = block_exclusive_scan<BLOCK, uint32_t>(c, L.wtmp, &total);
         phase_tick<COLLECT>(tacc, tmark, PT_COUNT);
-        if (ptotal) {
-            flush();
-            if (total == 0) phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
-        }
+        if (n_loaded) flush(n_loaded, (int)par);
+        // last round's staged rows into the registers (stage[par ^ 1] was
+        // written before this round's scan barriers)
+        load_rows(n_staged, (int)(par ^ 1u));
+        n_loaded = n_staged;
+        sb_loaded = sb_staged;
+        n_staged = 0;
         if (total == 0) {  // uniform
-            load_rows(0);
+            if (n_loaded) phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
             continue;
         }
         if (total <= (uint32_t)STAGE) {  // uniform
-            if (threadIdx.x == 0) psb = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
+            if (threadIdx.x == 0) sb_staged = atomicAdd((unsigned long long*)out_count, (unsigned long long)total);
             uint32_t w = pre;
 #pragma unroll
             for (int it = 0; it < PITEMS; it++) {
                 if (hitmask & (1u << it)) {
                     const uint32_t prow = base + (uint32_t)it * BLOCK + threadIdx.x;
-                    L.stage_p[w] = prow;
-                    L.stage_b[w] = L.hidx[q0[it]];
+                    L.stage_p[par][w] = prow;
+                    L.stage_b[par][w] = L.hidx[q0[it]];
                     w++;
                     if (multi & (1u << it)) {
                         for (uint32_t q = q0[it] + 1; q < qe[it]; q++) {
                             if (L.hkey[q] == kk[it]) {
-                                L.stage_p[w] = prow;
-                                L.stage_b[w] = L.hidx[q];
+                                L.stage_p[par][w] = prow;
+                                L.stage_b[par][w] = L.hidx[q];
                                 w++;
                             }
                         }
                     }
                 }
             }
-            __syncthreads();
-            load_rows(total);
-            ptotal = total;
+            n_staged = total;
             phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
             continue;
         }
@@ -697,14 +703,22 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
             }
         }
         // L.s_direct is rewritten only after the next round's scan barriers
-        load_rows(0);
         phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
     }
-    // the last staged round
-    if (ptotal) {
-        if (threadIdx.x == 0) L.s_base = psb;
+    // the rows still in flight: those in the registers (slot par) and those
+    // staged by the last round (stage[par ^ 1], slot par ^ 1)
+    if (n_loaded || n_staged) {  // uniform
+        __syncthreads();  // the last round's store read slot par ^ 1 after its scan
+        if (threadIdx.x == 0) {
+            if (n_loaded) L.s_base[par] = sb_loaded;
+            if (n_staged) L.s_base[par ^ 1u] = sb_staged;
+        }
         __syncthreads();
-        flush();
+        if (n_loaded) flush(n_loaded, (int)par);
+        if (n_staged) {
+            load_rows(n_staged, (int)(par ^ 1u));
+            flush(n_staged, (int)(par ^ 1u));
+        }
         phase_tick<COLLECT>(tacc, tmark, PT_WRITE);
     }
     // the caller synchronises before the next item rebuilds the table
