@@ -370,10 +370,11 @@ __device__ __forceinline__ bool prefilter_tile(const uint32_t* __restrict__ skey
 // The probe side (pn keys) streams through in rounds of BLOCK*PITEMS keys (the
 // next round is loaded while the current one is processed).  Each round counts
 // its matches and block-scans the counts.  A round with at most STAGE rows
-// stages them in LDS as (probe row, build row) pairs, reserves them with one
-// global atomic and loads their keys and values, one or two rows per thread;
-// the next round stores them after its own count (stores by consecutive
-// threads).  Larger rounds reserve and write from the lanes at once.  Both
+// stages them in LDS as (probe row, build row) pairs and reserves them with one
+// global atomic; the next round loads their keys and values, one or two rows
+// per thread, and the round after stores them after its own count (stores by
+// consecutive threads).  Larger rounds reserve and write from the lanes at
+// once.  Both
 // values of a row are read only for matches: the build side's from its
 // (cache-resident) chunk, the probe side's from its slice.  Rows past out_cap
 // are counted, not written.
