@@ -175,7 +175,9 @@ def run_batch(args, n_joins: int) -> dict:
     # its device time
     for p, _ in plain:
         p.collect_phase_times = 0
-    for p, d in plain[:4]:
+    # every join once first: the context's buffers grow to the largest join here,
+    # not inside the timed loop (a grown buffer is a hipFree + hipMalloc)
+    for p, d in plain:
         ctx.join(p, *d)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -80,6 +80,9 @@ int bj::ensure(DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (b.bytes >= bytes) return BJ_OK;
+    // a buffer that has to grow again grows by at least a quarter, so a caller
+    // whose joins keep getting larger reallocates a few times, not every call
+    if (b.p) bytes = std::max<size_t>(bytes, b.bytes + b.bytes / 4);
     if (b.p) {
         hipError_t e = hipFree(b.p);
         (void)e;

@@ -31,6 +31,11 @@ constexpr uint32_t B_MAX_SLICES = 16;
 constexpr uint64_t B_SEG_TARGET = 64 * 1024;     // Bloom segment built in LDS
 constexpr uint64_t B_SEG_MAX = 160 * 1024;
 constexpr int B_MAX_JOINS = 1 << 20;
+// A batch filters S before partitioning it, each key probing a random filter
+// word: fast while the join's filter stays in an XCD's L2, several times slower
+// from the MALL or HBM.  A filter chosen by the tuner or the default rule is
+// kept only up to this size.
+constexpr uint64_t B_FILTER_L2_BYTES = 3584ull << 10;
 
 struct JoinCfg {
     int bits = 1;
@@ -111,6 +116,8 @@ int resolve(const bj_ctx* c, const bj_join_spec& s, JoinCfg& o)
     if (bpk < 0.0) {
         const bj_tune_entry* t1 = find_tune(c, 1, 0, sr_class(nr, ns), sel_class(s.match_hint));
         bpk = t1 ? t1->bf_bits_per_key : (sel_class(s.match_hint) <= 1 ? 10.0 : 0.0);
+        // the tuner times its filters on joins whose filter fits the L2
+        if ((double)nr * bpk / 8.0 > (double)B_FILTER_L2_BYTES) bpk = 0.0;
     }
     if (nr == 0 || ns == 0) bpk = 0.0;
     o.bpk = bpk;
@@ -736,10 +743,13 @@ extern "C" int bj_autotune(bj_ctx* c, int n, const bj_join_spec* specs, int* ent
         best.sr = sr;
         best.sel = sel;
         best.ms = 1e30;
+        // a filter has to beat the join without one by 3%: near-ties otherwise
+        // flip between runs, and the filter is the riskier choice for the
+        // larger joins of the class
         for (double bpk : {0.0, 6.0, 10.0, 14.0}) {
             double ms = 0.0;
             if ((rc = time_config(c, d, bits, tile, bpk, ms))) break;
-            if (ms < best.ms) {
+            if (bpk == 0.0 ? ms < best.ms : ms < best.ms * 0.97) {
                 best.ms = ms;
                 best.bf_bits_per_key = bpk;
             }

@@ -29,7 +29,8 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
-// Grows b to at least bytes (contents are not kept).
+// Grows b to at least bytes (contents are not kept); a buffer that grows again
+// grows by at least a quarter of its size.
 int ensure(DevBuf& b, size_t bytes);
 void release(DevBuf& b);
 

@@ -49,6 +49,18 @@ def test_plan_selectivity_hint_rule():
     assert [c.bf_bits_per_key > 0 for c in cfg] == [True, True, False, False]
 
 
+def test_plan_filter_only_within_l2():
+    """A filter the planner chooses itself is kept only while it fits an XCD's L2
+    (3.5 MB): the batch probes it at random before partitioning S.  An explicit
+    filter is used as given."""
+    cfg, _ = bj.batch_plan([spec(2_800_000, 4 << 20, match_hint=0.01),   # 3.5 MB at 10 bits/key
+                            spec(3_000_000, 4 << 20, match_hint=0.01),   # 3.75 MB
+                            spec(16 << 20, 16 << 20, match_hint=0.01),
+                            spec(16 << 20, 16 << 20, match_hint=0.01, bf_bits_per_key=10.0)])
+    assert [c.bf_bits_per_key for c in cfg] == [10.0, 0.0, 0.0, 10.0]
+    assert [c.bf_k > 0 for c in cfg] == [True, False, False, True]
+
+
 @pytest.mark.parametrize("kw,code", [
     (dict(tile=1000), -1), (dict(radix_bits=16), -5), (dict(num_hashes=9, bf_bits_per_key=8.0), -1),
 ])
