@@ -149,7 +149,23 @@ __device__ __forceinline__ bool scatter_range(const ScatterLds& L, const uint32_
             }
         }
         __syncthreads();
-        if (small) {
+        if (nbk > 2 * WAVE && nbk <= BLOCK) {
+            // one bucket per thread, scanned by the whole workgroup: from 3
+            // buckets per lane on, one wave working through them serially keeps
+            // the others waiting longer than the scan's two extra barriers
+            const int b = threadIdx.x;
+            uint32_t c = 0;
+            if (b < nbk) {
+                scatter_retire(L, cprev, b);
+                c = cnt[b];
+            }
+            uint32_t tot;
+            const uint32_t ex = block_exclusive_scan<BLOCK, uint32_t>(c, L.wtmp, &tot);
+            if (b < nbk) {
+                L.off[b] = ex;
+                L.nw[b] = scatter_nw(L, b, c, cs);
+            }
+        } else if (small) {
             if (threadIdx.x < WAVE) {
                 // lane l owns buckets [l*per, l*per + per)
                 const int per = (nbk + WAVE - 1) / WAVE;
