@@ -333,3 +333,25 @@ def test_large_join_rows(gpu_ctx, kw):
     assert np.array_equal(np.sort(b.astype(np.int64)), expect_s)
     assert np.array_equal(skh[b], k)
     assert np.array_equal(dkh[a], k)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(probe_breakdown=1), dict(s_chunks=2), dict(tile=4096)])
+def test_probe_mixed_round_paths(gpu_ctx, kw):
+    """Rounds of one work item that alternate between staged rows (at most 1024
+    per round, stored two rounds later) and rows written from the lanes (a
+    round holding a key with 3000 build duplicates): the rows in flight across
+    a direct round, and at the end of each item, must all be written once."""
+    rng = np.random.default_rng(11)
+    heavy = np.uint32(0x01234567)
+    rk = np.concatenate([np.arange(1, 47_001, dtype=np.uint32) * np.uint32(40_009),
+                         np.full(3000, heavy, np.uint32)])
+    rv = np.arange(len(rk), dtype=np.uint32)
+    sk = rng.integers(1, 2**31, 2_000_000, dtype=np.uint64).astype(np.uint32)
+    sk[rng.choice(len(sk), 200_000, replace=False)] = rk[rng.integers(0, 47_000, 200_000)]
+    sk[rng.choice(len(sk), 25, replace=False)] = heavy
+    sv = np.arange(len(sk), dtype=np.uint32)
+    n, k, a, b = O.cpu_join(rk, rv, sk, sv)
+    assert n > 25 * 3000
+    res, rows = gpu_join(gpu_ctx, rk, rv, sk, sv, answer_scale=float(n) / len(sk) + 1.0, **kw)
+    assert res.match_count == n and res.truncated == 0
+    assert np.array_equal(rows, O.sort_rows(k, a, b))
