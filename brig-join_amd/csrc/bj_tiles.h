@@ -399,7 +399,7 @@ struct alignas(16) ProbeLds {
     static constexpr int STAGE = 2 * BLOCK;  // rows of one round staged in LDS
     uint32_t stage_p[2][STAGE];  // probe-side row of each of a round's rows, two
                                  // rounds (first: short LDS offsets)
-    uint16_t stage_b[2][STAGE];  // build-side row (in the chunk)
+    uint16_t stage_b[2][STAGE];  // table slot of the build tuple (its key and row)
     uint32_t hkey[CAP];        // keys by bucket (bucket counts while the table is built)
     uint16_t hidx[CAP];
     uint16_t bst[CAP + 2];     // bucket starts, CAP buckets at most
@@ -554,25 +554,27 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
     uint32_t n_staged = 0;          // rows staged last round, loaded this round (uniform)
     unsigned long long sb_loaded = 0, sb_staged = 0;  // thread 0: their reservations
     uint32_t par = 0;               // round parity
-    // The six loads are issued in every round, also where nothing is staged
-    // (rows 0 of both sides, unused): the loads in flight at the top of a round
-    // are then the same on every path (the next keys, then these six), so
-    // waiting for the keys never waits for these.
+    // The key and build row come from the table (LDS), the values from global
+    // memory.  The four loads are issued in every round, also where nothing is
+    // staged (row 0 of the probe side and table slot 0, unused): the loads in
+    // flight at the top of a round are then the same on every path (the next
+    // keys, then these four), so waiting for the keys never waits for these.
     auto load_rows = [&](uint32_t n, int buf) {
-        uint32_t p0 = 0, b0 = 0, p1 = 0, b1 = 0;
+        uint32_t p0 = 0, q0s = 0, p1 = 0, q1s = 0;
         if (n) {  // uniform; rows tid and tid + BLOCK, clamped to the last row
             const uint32_t j0 = min((uint32_t)threadIdx.x, n - 1u);
             const uint32_t j1 = min((uint32_t)threadIdx.x + BLOCK, n - 1u);
             p0 = L.stage_p[buf][j0];
-            b0 = L.stage_b[buf][j0];
+            q0s = L.stage_b[buf][j0];
             p1 = L.stage_p[buf][j1];
-            b1 = L.stage_b[buf][j1];
+            q1s = L.stage_b[buf][j1];
         }
+        wk0 = L.hkey[q0s];
+        wk1 = L.hkey[q1s];
+        uint32_t b0 = L.hidx[q0s], b1 = L.hidx[q1s];
         // per-lane rows: vector loads on every path (a known row 0 would become
         // a scalar load, which the vector load counter does not see)
         asm volatile("" : "+v"(p0), "+v"(b0), "+v"(p1), "+v"(b1));
-        wk0 = pkp[p0];
-        wk1 = pkp[p1];
         wp0 = pvp[p0];
         wp1 = pvp[p1];
         wb0 = bvp[b0];
@@ -661,13 +663,13 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
                 if (hitmask & (1u << it)) {
                     const uint32_t prow = base + (uint32_t)it * BLOCK + threadIdx.x;
                     L.stage_p[par][w] = prow;
-                    L.stage_b[par][w] = L.hidx[q0[it]];
+                    L.stage_b[par][w] = (uint16_t)q0[it];
                     w++;
                     if (multi & (1u << it)) {
                         for (uint32_t q = q0[it] + 1; q < qe[it]; q++) {
                             if (L.hkey[q] == kk[it]) {
                                 L.stage_p[par][w] = prow;
-                                L.stage_b[par][w] = L.hidx[q];
+                                L.stage_b[par][w] = (uint16_t)q;
                                 w++;
                             }
                         }
