@@ -246,28 +246,34 @@ __global__ __launch_bounds__(1024) void k_starts(const uint32_t* __restrict__ to
 
 // First-level cursors of the pass-1 scatter: l1off[g][b1] = start of bucket b1
 // (sb_start[b1 << bits2]) + sum_{g' < g} l1h[g'][b1].  A column scan: each
-// workgroup owns 64 columns; 16 row groups per column are summed, scanned in
-// LDS, then rescanned row by row.  All loads/stores are coalesced across columns.
+// workgroup owns 16 columns; 64 row ranges per column are summed, scanned by
+// one wave per column, then rescanned row by row.  Short row ranges keep the
+// serial chains short when there are few columns (128 first-level buckets give
+// 8 workgroups); loads and stores are coalesced over a row's 16 columns.
+constexpr int L1_COLS = 16, L1_ROWS = 1024 / L1_COLS;
+static_assert(L1_ROWS == WAVE && 1024 / WAVE == L1_COLS, "one wave scans one column");
+
 __global__ __launch_bounds__(1024) void k_l1_offsets(const uint32_t* __restrict__ l1h, int G, int P1,
                                                      int bits2, const uint64_t* __restrict__ sb_start,
                                                      uint64_t* __restrict__ l1off)
 {
-    __shared__ uint64_t part_sum[16][64];
-    const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
-    const int b1 = blockIdx.x * 64 + c;
-    const int rows = (G + 15) / 16;
+    __shared__ uint64_t part_sum[L1_ROWS][L1_COLS + 1];
+    const int c = threadIdx.x % L1_COLS, r = threadIdx.x / L1_COLS;
+    const int b1 = blockIdx.x * L1_COLS + c;
+    const int rows = (G + L1_ROWS - 1) / L1_ROWS;
     const int g0 = min(r * rows, G), g1 = min(g0 + rows, G);
     uint64_t s = 0;
     if (b1 < P1)
         for (int g = g0; g < g1; g++) s += l1h[(size_t)g * P1 + b1];
     part_sum[r][c] = s;
     __syncthreads();
-    if (r == 0 && b1 < P1) {
-        uint64_t acc = sb_start[(size_t)b1 << bits2];
-        for (int q = 0; q < 16; q++) {
-            const uint64_t v = part_sum[q][c];
-            part_sum[q][c] = acc;
-            acc += v;
+    {
+        // wave w: column w, lane = row range (the branch is uniform per wave)
+        const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+        const int col = blockIdx.x * L1_COLS + w;
+        if (col < P1) {
+            const uint64_t v = part_sum[lane][w];
+            part_sum[lane][w] = sb_start[(size_t)col << bits2] + wave_inclusive_scan<uint64_t>(v) - v;
         }
     }
     __syncthreads();
@@ -961,7 +967,7 @@ hipError_t launch_offsets(const uint32_t* part, int G, int bits, int bits2, int 
     const int nbins = 1 << bits, P1 = 1 << (bits - bits2);
     k_colsums<<<ceil_div_i(nbins, 1024) * W, 1024, 0, st>>>(part, G, bits, W, colsum, totals);
     k_starts<<<1, 1024, 0, st>>>(totals, nbins, sb_start);
-    k_l1_offsets<<<ceil_div_i(P1, 64), 1024, 0, st>>>(bits2 ? l1h : part, G, P1, bits2, sb_start, l1off);
+    k_l1_offsets<<<ceil_div_i(P1, L1_COLS), 1024, 0, st>>>(bits2 ? l1h : part, G, P1, bits2, sb_start, l1off);
     return hipGetLastError();
 }
 
