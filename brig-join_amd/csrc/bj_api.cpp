@@ -514,7 +514,7 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
     uint64_t* item_start = P<uint64_t>(c->item_start);
     BJ_HIP(beg(BJ_K_PLAN));
     BJ_HIP(launch_plan(R.start, P<uint64_t>(c->s_start), s_cnt, pl.bits, (uint32_t)pl.tile, pl.slice,
-                       pl.max_slices, P<uint32_t>(c->plan_cnt), item_start, P<uint64_t>(c->partials),
+                       pl.max_slices, P<uint32_t>(c->plan_cnt), item_start, ctr + CTR_ITEMS,
                        P<HeavyItem>(c->items), pl.item_cap, ctr + CTR_ERR, st));
     BJ_HIP(end(BJ_K_PLAN));
     BJ_HIP(beg(BJ_K_PROBE));
@@ -988,16 +988,13 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
     if ((rc = enqueue_s_probe(c, pl, R, s_key, s_val, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
                               P<uint32_t>(c->out_sval), pl.cap)))
         return rc;
-    const int nbins = 1 << pl.bits;
-    uint64_t* item_start = P<uint64_t>(c->item_start);
 
     BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    BJ_HIP(hipMemcpyAsync(c->h_counters + CTR_N, item_start + nbins, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     BJ_HIP(hipStreamSynchronize(st));
 
     const uint64_t* h = c->h_counters;
     if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
-    fill_result(c, pl, h, pl.bf ? h[CTR_SPRIME] : pl.ns, h[CTR_N], out);
+    fill_result(c, pl, h, pl.bf ? h[CTR_SPRIME] : pl.ns, h[CTR_ITEMS], out);
     out->s_chunks = 1;
     if (timing) {
         double* k = out->kernel_ms;

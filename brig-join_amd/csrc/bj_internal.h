@@ -41,6 +41,7 @@ uint64_t splitmix64(uint64_t x);
 PermKeys perm_keys(uint64_t seed);
 int ceil_log2(uint64_t n);
 
+// CTR_ITEMS: probe work items of the last plan (written by the plan kernel);
 // CTR_XCD .. CTR_XCD + 7: tile counters of the filter kernel, one per XCD;
 // CTR_ACC_SPRIME / CTR_ACC_ITEMS: |S'| and vlsplit items summed over the chunks of S
 enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_ACC_SPRIME, CTR_ACC_ITEMS, CTR_XCD = 8, CTR_N = 16 };

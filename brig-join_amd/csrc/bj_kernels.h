@@ -66,8 +66,8 @@ hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const ui
                             uint64_t* tile_ctr, uint64_t* err, int num_cu, hipStream_t st);
 hipError_t launch_plan(const uint64_t* r_start, const uint64_t* s_start, const uint32_t* s_cnt, int bits,
                        uint32_t tile, uint32_t slice, uint32_t max_slices, uint32_t* counts,
-                       uint64_t* item_start, uint64_t* partials, HeavyItem* items, uint64_t item_cap,
-                       uint64_t* err, hipStream_t st);
+                       uint64_t* item_start, uint64_t* total_out, HeavyItem* items, uint64_t item_cap,
+                       uint64_t* err, hipStream_t st);  // total_out: item count (nullable)
 hipError_t launch_probe(int tile, const uint32_t* rk, const uint32_t* rv, const uint64_t* r_start,
                         const uint32_t* sk, const uint32_t* sv, const uint64_t* s_start,
                         const uint32_t* s_cnt, int bits, uint32_t slice, const HeavyItem* items,

@@ -713,15 +713,16 @@ __global__ void k_plan_count(const uint64_t* __restrict__ r_start, const uint64_
 }
 
 // One workgroup: thread t owns a run of at most 32 sub-buckets; one block scan
-// of the counts into item_start (item_start[nbins] = total), then the items of
-// the (few) split sub-buckets.
+// of the counts into item_start (item_start[nbins] = total, also stored at
+// *total_out when given), then the items of the (few) split sub-buckets.
 __global__ __launch_bounds__(1024) void k_plan_fill(const uint64_t* __restrict__ r_start,
                                                     const uint64_t* __restrict__ s_start,
                                                     const uint32_t* __restrict__ s_cnt, int nbins, uint32_t tile,
                                                     uint32_t slice, uint32_t max_slices,
                                                     const uint32_t* __restrict__ n_items,
                                                     uint64_t* __restrict__ item_start, uint64_t item_cap,
-                                                    HeavyItem* __restrict__ items, uint64_t* __restrict__ err)
+                                                    HeavyItem* __restrict__ items, uint64_t* __restrict__ err,
+                                                    uint64_t* __restrict__ total_out)
 {
     // nbins is a power of two up to 2^15: thread t owns sub-buckets
     // [t * per, (t + 1) * per), per = nbins / 1024 (at least 1); counts read as
@@ -752,7 +753,10 @@ __global__ __launch_bounds__(1024) void k_plan_fill(const uint64_t* __restrict__
     for (int i = 0; i < MAXPER; i++) sum += v[i];
     uint64_t total;
     uint64_t base = block_exclusive_scan<1024, uint64_t>(sum, wtmp, &total);
-    if (threadIdx.x == 0) item_start[nbins] = total;
+    if (threadIdx.x == 0) {
+        item_start[nbins] = total;
+        if (total_out) *total_out = total;
+    }
     if (per >= 4) {
         ulonglong2* o2 = (ulonglong2*)(item_start + b0);
         uint64_t e = base;
@@ -1082,16 +1086,15 @@ hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const ui
 
 hipError_t launch_plan(const uint64_t* r_start, const uint64_t* s_start, const uint32_t* s_cnt, int bits,
                        uint32_t tile, uint32_t slice, uint32_t max_slices, uint32_t* counts,
-                       uint64_t* item_start, uint64_t* partials, HeavyItem* items, uint64_t item_cap,
+                       uint64_t* item_start, uint64_t* total_out, HeavyItem* items, uint64_t item_cap,
                        uint64_t* err, hipStream_t st)
 {
-    (void)partials;
     if (bits > 15) return hipErrorInvalidValue;
     const int nbins = 1 << bits;
     k_plan_count<<<ceil_div_i(nbins, 256), 256, 0, st>>>(r_start, s_start, s_cnt, nbins, tile, slice, max_slices,
                                                          counts);
     k_plan_fill<<<1, 1024, 0, st>>>(r_start, s_start, s_cnt, nbins, tile, slice, max_slices, counts, item_start,
-                                    item_cap, items, err);
+                                    item_cap, items, err, total_out);
     return hipGetLastError();
 }
 
