@@ -935,8 +935,8 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
 
 extern "C" {
 
-int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t* r_val,
-            const uint32_t* s_key, const uint32_t* s_val, bj_result* out)
+static int join_once(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t* r_val,
+                     const uint32_t* s_key, const uint32_t* s_val, bj_result* out)
 {
     if (!c || !p || !out) return fail(BJ_ERR_INVALID, "null argument");
     std::memset(out, 0, sizeof(*out));
@@ -1002,6 +1002,19 @@ int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t
         fill_phases(out, h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0, tl_span_ms(c, BJ_K_HIST_R, BJ_K_PROBE));
     }
     return probe_timers_end(c, pl, out);
+}
+
+int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t* r_val,
+            const uint32_t* s_key, const uint32_t* s_val, bj_result* out)
+{
+    const int rc = join_once(c, p, r_key, r_val, s_key, s_val, out);
+    if (rc != BJ_OK && c && c->stream) {
+        // an error return leaves no copy of the caller's buffers and no kernel
+        // on them in flight: the caller may free them once this returns
+        if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    return rc;
 }
 
 int bj_partition(bj_ctx* c, const bj_params* p, const uint32_t* key, const uint32_t* val, int64_t n,

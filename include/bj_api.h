@@ -201,7 +201,9 @@ int  bj_plan(const bj_params *p, bj_plan_info *out);
 
 /* ---- the join ---------------------------------------------------------- */
 /* Equi-join R(key, r_val) with S(key, s_val) on key.  Emits every matching
- * (key, r_val, s_val) row.  Sizes come from p->r_size / p->s_size. */
+ * (key, r_val, s_val) row.  Sizes come from p->r_size / p->s_size.  Returns
+ * after the join has finished; an error return also waits for the work
+ * already queued, so the input buffers are free to reuse either way. */
 int  bj_join(bj_ctx *ctx, const bj_params *p,
              const uint32_t *r_key, const uint32_t *r_val,
              const uint32_t *s_key, const uint32_t *s_val,

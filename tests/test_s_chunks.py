@@ -92,6 +92,20 @@ def test_more_chunks_than_tuples(gpu_ctx):
         assert rows.tolist() == [[9, 2, 10], [0x80000001, 3, 12]]
 
 
+def test_context_after_rejected_call(gpu_ctx):
+    """An error return drains the context's streams; the next chunked join on
+    the same context, from host buffers, is exact."""
+    rk = np.array([5, 7, 9, 0x80000001], np.uint32)
+    rv = np.arange(4, dtype=np.uint32)
+    sk = np.array([9, 1, 0x80000001, 7], np.uint32)
+    sv = np.array([10, 11, 12, 13], np.uint32)
+    with pytest.raises(bj.BjError):
+        run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=-1)
+    res, rows = run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=2, use_bf=1)
+    assert res.s_chunks == 2 and res.match_count == 3
+    assert rows.tolist() == [[7, 1, 13], [9, 2, 10], [0x80000001, 3, 12]]
+
+
 def test_duplicates_across_chunks(gpu_ctx):
     """Skewed S (every key of a small set, many times) with duplicate R keys."""
     rng = np.random.default_rng(7)
