@@ -806,6 +806,11 @@ __global__ __launch_bounds__(1024) void k_plan_fill(const uint64_t* __restrict__
 // past it (probe_item, bj_tiles.h).
 // COLLECT (the reference's COLLECT / --fused_probe_breakdown): thread 0 of
 // every workgroup adds its phase times (PT_*) to timers[blockIdx.x * PT_N + p].
+#ifndef BJ_PROBE_GRAB
+#define BJ_PROBE_GRAB 2
+#endif
+constexpr int PROBE_GRAB = BJ_PROBE_GRAB;
+
 template <int BLOCK, int CAP, int PITEMS, bool COLLECT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_probe(
     const uint32_t* __restrict__ rk, const uint32_t* __restrict__ rv,
@@ -826,13 +831,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     const uint64_t n_heavy = min(*n_items_p, item_cap);
     const uint64_t n_work = n_heavy + (uint64_t)nbins;
 
-    if (threadIdx.x == 0) s_item = atomicAdd((unsigned long long*)work_ctr, 1ull);
+    // items are claimed PROBE_GRAB at a time: thread 0 waits for the queue's
+    // atomic once per claim instead of once per item
+    if (threadIdx.x == 0) s_item = atomicAdd((unsigned long long*)work_ctr, (unsigned long long)PROBE_GRAB);
     __syncthreads();
     uint64_t item = s_item;
+    int left = PROBE_GRAB;  // items of the current claim, this one included
     __syncthreads();
 
     while (item < n_work) {
-        if (threadIdx.x == 0) s_next = atomicAdd((unsigned long long*)work_ctr, 1ull);
+        if (left == 1 && threadIdx.x == 0)
+            s_next = atomicAdd((unsigned long long*)work_ctr, (unsigned long long)PROBE_GRAB);
 
         uint32_t sb, chunk = 0, sl = 0, nsl = 1;
         const bool heavy = item < n_heavy;
@@ -863,7 +872,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                                                     tacc, &tmark);
         }
         __syncthreads();  // the table is rebuilt by the next item; s_next is written
-        item = s_next;
+        if (left > 1) {
+            item++;
+            left--;
+        } else {
+            item = s_next;
+            left = PROBE_GRAB;
+        }
         __syncthreads();
     }
     if (COLLECT && threadIdx.x == 0) {
