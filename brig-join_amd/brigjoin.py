@@ -9,12 +9,14 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libbrigjoin.so")
 DRIVER_PATH = os.path.join(HERE, "bin", "bj_driver")
+GENDATA_PATH = os.path.join(HERE, "bin", "bj_gendata")
 
 BJ_OK = 0
 ERRORS = {-1: "BJ_ERR_INVALID", -2: "BJ_ERR_HIP", -3: "BJ_ERR_NOMEM", -4: "BJ_ERR_IO",
@@ -35,7 +37,8 @@ EXPORTS = [
     "bj_server_opts_default", "bj_server_create", "bj_server_add_relation", "bj_server_request_bytes",
     "bj_server_submit", "bj_server_poll", "bj_server_wait", "bj_server_copy_rows", "bj_server_release",
     "bj_server_drain",
-    "bj_server_stats_get", "bj_server_destroy",
+    "bj_server_stats_get", "bj_server_destroy", "bj_gen_refdist_host", "bj_gen_refdist",
+    "bj_join_noalloc",
 ]
 
 
@@ -245,6 +248,13 @@ def lib():
         L.bj_server_drain.argtypes = [vp]
         L.bj_server_stats_get.argtypes = [vp, ctypes.POINTER(BjServerStats)]
         L.bj_server_destroy.argtypes = [vp]
+        L.bj_join_noalloc.argtypes = [vp, ctypes.POINTER(BjParams), vp, vp, vp, vp, vp, vp, vp,
+                                      ctypes.c_uint64, ctypes.POINTER(BjResult)]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.bj_gen_refdist_host.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64,
+                                          ctypes.c_double, vp, vp, i64p]
+        L.bj_gen_refdist.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64,
+                                     ctypes.c_double, vp, vp, i64p]
         _lib = L
     return _lib
 
@@ -303,6 +313,10 @@ class Context:
         self.close()
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be going away:
+        # leave the memory to the end of the process instead of freeing it
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -316,6 +330,16 @@ class Context:
         rc = lib().bj_join(self.h, ctypes.byref(params), _ptr(r_key), _ptr(r_val),
                            _ptr(s_key), _ptr(s_val), ctypes.byref(res))
         check(rc, "bj_join")
+        return res
+
+    def join_noalloc(self, params: BjParams, r_key, r_val, s_key, s_val, out_key, out_rval, out_sval,
+                     out_cap: int) -> BjResult:
+        """bj_join_noalloc: device inputs, rows into the caller's device arrays."""
+        res = BjResult()
+        rc = lib().bj_join_noalloc(self.h, ctypes.byref(params), _ptr(r_key), _ptr(r_val), _ptr(s_key),
+                                   _ptr(s_val), _ptr(out_key), _ptr(out_rval), _ptr(out_sval), out_cap,
+                                   ctypes.byref(res))
+        check(rc, "bj_join_noalloc")
         return res
 
     def partition(self, params: BjParams, key, val, n: int, out_key, out_val, out_start) -> None:
@@ -338,6 +362,24 @@ class Context:
                   zipf_s: float = 0.0) -> None:
         check(lib().bj_gen_probe(self.h, seed_r, seed_s, n_r, n_s, m_s, zipf_s, _ptr(d_key),
                                  _ptr(d_val)), "bj_gen_probe")
+
+    def gen_refdist(self, r_size: int, s_size: int, answer_pct: float, d_r_key, d_s_key, seed: int = 42,
+                    max_pct: float = -1.0) -> int:
+        """The reference datagen's key distribution on the device; returns the match count."""
+        k = ctypes.c_int64()
+        check(lib().bj_gen_refdist(self.h, r_size, s_size, answer_pct, seed, max_pct, _ptr(d_r_key),
+                                   _ptr(d_s_key), ctypes.byref(k)), "bj_gen_refdist")
+        return int(k.value)
+
+
+def gen_refdist_host(r_size: int, s_size: int, answer_pct: float, seed: int = 42, max_pct: float = -1.0):
+    """The reference datagen's keys in its exact order (host); returns (r_key, s_key, matches)."""
+    rk = np.empty(max(r_size, 1), np.uint32)
+    sk = np.empty(max(s_size, 1), np.uint32)
+    k = ctypes.c_int64()
+    check(lib().bj_gen_refdist_host(r_size, s_size, answer_pct, seed, max_pct, rk.ctypes.data, sk.ctypes.data,
+                                    ctypes.byref(k)), "bj_gen_refdist_host")
+    return rk[:r_size], sk[:s_size], int(k.value)
 
 
 def join_spec(r_key=None, r_val=None, s_key=None, s_val=None, r_size=0, s_size=0, **kw) -> BjJoinSpec:
@@ -405,6 +447,10 @@ class Batch:
         self.close()
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be going away:
+        # leave the memory to the end of the process instead of freeing it
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -511,6 +557,10 @@ class Server:
         self.close()
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be going away:
+        # leave the memory to the end of the process instead of freeing it
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

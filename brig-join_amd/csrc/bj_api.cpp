@@ -76,10 +76,13 @@ bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
 
 }  // namespace
 
+thread_local bool bj::g_no_alloc = false;
+
 int bj::ensure(DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (b.bytes >= bytes) return BJ_OK;
+    if (g_no_alloc) return fail(BJ_ERR_INTERNAL, "device allocation inside the no-allocation entry point");
     // a buffer that has to grow again grows by at least a quarter, so a caller
     // whose joins keep getting larger reallocates a few times, not every call
     if (b.p) bytes = std::max<size_t>(bytes, b.bytes + b.bytes / 4);
@@ -413,6 +416,10 @@ void tl_reset(bj_ctx* c, bool timing)
 static hipError_t tl_mark(bj_ctx* c, int& idx)
 {
     if (c->tl_next >= (int)c->ev.size()) {
+        if (g_no_alloc) {  // no new events there: the slot reads as 0 ms
+            idx = -1;
+            return hipSuccess;
+        }
         hipEvent_t e = nullptr;
         if (hipEventCreate(&e) != hipSuccess) {
             (void)hipGetLastError();
@@ -596,6 +603,9 @@ int bj_create(bj_ctx** out, int device)
     c->max_lds = max_lds;
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = kernels_init(max_lds);
+    // the scratch array of bj_join_noalloc on this device (its failure only
+    // disables that entry point, which then reports it)
+    if (e == hipSuccess) (void)noalloc_init(device);
     if (e == hipSuccess) e = batch_kernels_init(max_lds);
     c->ev.assign(EV_N, nullptr);
     for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
@@ -931,6 +941,42 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
     return probe_timers_end(c, cp, out);
 }
 
+// The join of device-resident inputs in one pass, into the row arrays o_*
+// (capacity pl.cap): the workspace must already be in place.  h2d_ms < 0: the
+// host copies before it were bracketed by EV_H2D0 / EV_H2D1.
+int join_device(bj_ctx* c, const Plan& pl, bool timing, double h2d_ms, const uint32_t* r_key,
+                const uint32_t* r_val, const uint32_t* s_key, const uint32_t* s_val, uint32_t* o_key,
+                uint32_t* o_rval, uint32_t* o_sval, bj_result* out)
+{
+    int rc;
+    hipStream_t st = c->stream;
+    uint64_t* ctr = P<uint64_t>(c->counters);
+    BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
+    if ((rc = probe_timers_begin(c, pl))) return rc;
+    // slots without kernels (no filter) keep no events and read as 0 ms
+    if ((rc = enqueue_r(c, pl, r_key, r_val))) return rc;
+    const RSide R{P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start), P<uint64_t>(c->filter)};
+    if ((rc = enqueue_s_probe(c, pl, R, s_key, s_val, o_key, o_rval, o_sval, pl.cap))) return rc;
+
+    BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    BJ_HIP(hipStreamSynchronize(st));
+
+    const uint64_t* h = c->h_counters;
+    if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
+    fill_result(c, pl, h, pl.bf ? h[CTR_SPRIME] : pl.ns, h[CTR_ITEMS], out);
+    out->key = o_key;
+    out->r_val = o_rval;
+    out->s_val = o_sval;
+    out->s_chunks = 1;
+    if (timing) {
+        double* k = out->kernel_ms;
+        for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = tl_ms(c, i);
+        fill_phases(out, h2d_ms < 0 ? ev_ms(c, EV_H2D0, EV_H2D1) : h2d_ms,
+                    tl_span_ms(c, BJ_K_HIST_R, BJ_K_PROBE));
+    }
+    return probe_timers_end(c, pl, out);
+}
+
 }  // namespace bj
 
 extern "C" {
@@ -978,30 +1024,9 @@ static int join_once(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const
         s_val = P<uint32_t>(c->in_sv);
     }
     if (h2d_ev) BJ_HIP(hipEventRecord(c->ev[EV_H2D1], st));
-
-    uint64_t* ctr = P<uint64_t>(c->counters);
-    BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
-    if ((rc = probe_timers_begin(c, pl))) return rc;
-    // slots without kernels (no filter) keep no events and read as 0 ms
-    if ((rc = enqueue_r(c, pl, r_key, r_val))) return rc;
-    const RSide R{P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start), P<uint64_t>(c->filter)};
-    if ((rc = enqueue_s_probe(c, pl, R, s_key, s_val, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
-                              P<uint32_t>(c->out_sval), pl.cap)))
-        return rc;
-
-    BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    BJ_HIP(hipStreamSynchronize(st));
-
-    const uint64_t* h = c->h_counters;
-    if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
-    fill_result(c, pl, h, pl.bf ? h[CTR_SPRIME] : pl.ns, h[CTR_ITEMS], out);
-    out->s_chunks = 1;
-    if (timing) {
-        double* k = out->kernel_ms;
-        for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = tl_ms(c, i);
-        fill_phases(out, h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0, tl_span_ms(c, BJ_K_HIST_R, BJ_K_PROBE));
-    }
-    return probe_timers_end(c, pl, out);
+    const double h2d_ms = h2d_ev ? -1.0 : 0.0;
+    return join_device(c, pl, timing, h2d_ms, r_key, r_val, s_key, s_val, P<uint32_t>(c->out_key),
+                       P<uint32_t>(c->out_rval), P<uint32_t>(c->out_sval), out);
 }
 
 int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t* r_val,

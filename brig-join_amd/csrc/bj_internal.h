@@ -33,9 +33,14 @@ struct DevBuf {
 // grows by at least a quarter of its size.
 int ensure(DevBuf& b, size_t bytes);
 void release(DevBuf& b);
+// Set on a host thread while it runs the no-allocation entry point: ensure()
+// then refuses to allocate, and no timing events are created.
+extern thread_local bool g_no_alloc;
 
 template <typename T>
 T* P(DevBuf& b) { return (T*)b.p; }
+template <typename T>
+const T* P(const DevBuf& b) { return (const T*)b.p; }
 
 uint64_t splitmix64(uint64_t x);
 PermKeys perm_keys(uint64_t seed);
@@ -132,6 +137,14 @@ int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* 
 // radix bits, then the build + probe against R; rows to o_* (capacity cap).
 int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
                     uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, uint64_t cap);
+// One-pass join of device-resident inputs into o_* (capacity pl.cap), with the
+// workspace already in place; h2d_ms < 0: host copies bracketed by EV_H2D0/1.
+int join_device(bj_ctx* c, const Plan& pl, bool timing, double h2d_ms, const uint32_t* r_key,
+                const uint32_t* r_val, const uint32_t* s_key, const uint32_t* s_val, uint32_t* o_key,
+                uint32_t* o_rval, uint32_t* o_sval, bj_result* out);
+// Looks up the no-allocation entry point's scratch array on `device` (the current
+// device), once per device (bj_create calls it).
+hipError_t noalloc_init(int device);
 float ev_ms(bj_ctx* c, int a, int b);
 // Kernel-slot timeline (see bj_ctx::tl_*).
 void tl_reset(bj_ctx* c, bool timing);

@@ -27,6 +27,13 @@ hipError_t launch_gen_s(uint32_t* key, uint32_t* val, uint64_t n_s, uint64_t n_r
                         uint32_t useed, double zipf_s, uint32_t zseed, double* d_tmp,
                         hipStream_t st);
 
+// The reference datagen's distribution (datagen/gen_dataset.cpp:172-220) in a
+// keyed random order: R of n keys with k matching ones; one region of S (n keys,
+// m matching keys from mc0 on, S-only keys from offset sj0 of their sequence).
+hipError_t launch_gen_refdist_r(uint32_t* key, uint64_t n, uint64_t k, const PermKeys& K, hipStream_t st);
+hipError_t launch_gen_refdist_s(uint32_t* key, uint64_t n, uint64_t m, uint64_t mc0, uint64_t sj0, uint64_t n_r,
+                                const PermKeys& K, hipStream_t st);
+
 // Also zeroes totals (2^bits entries) for launch_offsets when not null, and
 // with bits2 > 0 writes each chunk's first-level counts l1h[g][b1] (the sums
 // over the low bits2 bits).

@@ -120,6 +120,44 @@ __global__ void k_gen_s(uint32_t* __restrict__ key, uint32_t* __restrict__ val, 
     }
 }
 
+// The reference datagen's distribution (datagen/gen_dataset.cpp:172-220) on the
+// device.  Before its shuffle, R is [0, k) then k + (i - k) mod (2^31 - 1 - k),
+// and S is a run of matching keys (0, 1, ...) and S-only keys R + j mod
+// (2^31 - 1 - R) per region; the reference shuffles each region with
+// Fisher-Yates, here position i of a region takes the unshuffled element at a
+// keyed permutation of i (cycle-walked onto the region), so each region holds
+// the same key multiset in another random order.
+__device__ __forceinline__ uint32_t perm_onto(uint64_t i, uint64_t n, int L, const PermKeys& K)
+{
+    uint32_t p = permw((uint32_t)i, L, K);
+    while ((uint64_t)p >= n) p = permw(p, L, K);
+    return p;
+}
+
+__global__ void k_gen_refdist_r(uint32_t* __restrict__ key, uint64_t n, uint64_t k, int L, PermKeys K)
+{
+    const uint64_t r_only = 0x7FFFFFFFull - k;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = perm_onto(i, n, L, K);
+        key[i] = (uint32_t)(t < k ? t : k + (t - k) % r_only);
+    }
+}
+
+// One region of S: positions [0, n) of key (already offset), the first m
+// unshuffled elements matching keys mc0, mc0 + 1, ..., the rest S-only keys
+// n_r + (sj0 + j) mod (2^31 - 1 - n_r).
+__global__ void k_gen_refdist_s(uint32_t* __restrict__ key, uint64_t n, uint64_t m, uint64_t mc0,
+                                uint64_t sj0, uint64_t n_r, int L, PermKeys K)
+{
+    const uint64_t s_only = 0x7FFFFFFFull - n_r;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = perm_onto(i, n, L, K);
+        key[i] = (uint32_t)(t < m ? mc0 + t : n_r + (sj0 + (t - m)) % s_only);
+    }
+}
+
 __global__ void k_zipf_Hn(double* out, uint32_t n, double s)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = zipf_H((double)n, s);
@@ -926,6 +964,30 @@ hipError_t launch_gen_s(uint32_t* key, uint32_t* val, uint64_t n_s, uint64_t n_r
     }
     const int grid = (int)std::min<uint64_t>(65536, (n_s + 255) / 256);
     k_gen_s<<<grid, 256, 0, st>>>(key, val, n_s, n_r, m_s, L, KR, KP, KN, useed, zipf_s, Hn, zseed);
+    return hipGetLastError();
+}
+
+static int log2_ceil_u64(uint64_t n)
+{
+    int l = 0;
+    while ((1ull << l) < n) l++;
+    return l;
+}
+
+hipError_t launch_gen_refdist_r(uint32_t* key, uint64_t n, uint64_t k, const PermKeys& K, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = (int)std::min<uint64_t>(65536, (n + 255) / 256);
+    k_gen_refdist_r<<<grid, 256, 0, st>>>(key, n, k, std::max(1, log2_ceil_u64(n)), K);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_refdist_s(uint32_t* key, uint64_t n, uint64_t m, uint64_t mc0, uint64_t sj0, uint64_t n_r,
+                                const PermKeys& K, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const int grid = (int)std::min<uint64_t>(65536, (n + 255) / 256);
+    k_gen_refdist_s<<<grid, 256, 0, st>>>(key, n, m, mc0, sj0, n_r, std::max(1, log2_ceil_u64(n)), K);
     return hipGetLastError();
 }
 

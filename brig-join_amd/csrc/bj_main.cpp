@@ -177,7 +177,7 @@ void bj_main_args_default(bj_main_args* a)
     a->answer_scale = 1.0;
     a->radix_bits = -1;
     a->bf_bits_per_key = 0.0;
-    a->val_mode = 0;
+    a->val_mode = 1;  // the reference's synthesized payloads (src/hash_kernel.cu:330-336)
     a->device = 0;
     a->verbose = 1;
 }

@@ -99,6 +99,7 @@ using Item = std::pair<DevBuf bj_ctx::*, uint64_t>;
 struct Request {
     uint64_t id = 0;
     int rel = 0;
+    const Relation* relp = nullptr;  // relations are heap objects that stay put
     const uint32_t* sk = nullptr;
     const uint32_t* sv = nullptr;
     Plan pl;
@@ -139,7 +140,9 @@ struct bj_server {
     DevBuf arena;
     ArenaAlloc alloc;
     std::vector<Slot> slots;
-    std::mutex mu;
+    // guards rels, the queue, reqs and the counters below (rels only grows; its
+    // elements are heap objects, so a pointer taken under the lock stays valid)
+    mutable std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
     std::map<uint64_t, Request*> reqs;
@@ -179,6 +182,13 @@ int make_request_plan(const bj_server* s, const Relation& R, uint64_t ns, Plan& 
     return BJ_OK;
 }
 
+// The relation with index rel (null if there is none), looked up under the lock.
+const Relation* find_rel(const bj_server* s, int32_t rel)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    return (rel >= 0 && rel < (int32_t)s->rels.size()) ? s->rels[rel] : nullptr;
+}
+
 // Starts a request on a slot (scheduler thread, lock held).
 int launch(bj_server* s, Slot& sl, Request* q)
 {
@@ -190,7 +200,7 @@ int launch(bj_server* s, Slot& sl, Request* q)
         (c->*(it.first)).bytes = it.second;
         off += it.second;
     }
-    Relation& R = *s->rels[q->rel];
+    const Relation& R = *q->relp;
     const RSide rs{P<uint32_t>(R.key), P<uint32_t>(R.val), P<uint64_t>(R.start),
                    R.plan.bf ? P<uint64_t>(R.filter) : nullptr};
     const uint64_t col = align_up(q->cap * 4);
@@ -486,9 +496,10 @@ int bj_server_add_relation(bj_server* s, const uint32_t* r_key, const uint32_t* 
 int bj_server_request_bytes(const bj_server* s, int32_t rel, int64_t n_s, uint64_t* bytes)
 {
     if (!s || !bytes || n_s < 0) return fail(BJ_ERR_INVALID, "bad argument");
-    if (rel < 0 || rel >= (int32_t)s->rels.size()) return fail(BJ_ERR_INVALID, "no such relation");
+    const Relation* R = find_rel(s, rel);
+    if (!R) return fail(BJ_ERR_INVALID, "no such relation");
     Plan pl;
-    int rc = make_request_plan(s, *s->rels[rel], (uint64_t)n_s, pl);
+    int rc = make_request_plan(s, *R, (uint64_t)n_s, pl);
     if (rc) return rc;
     std::vector<Item> items;
     uint64_t ws = 0;
@@ -502,9 +513,10 @@ int bj_server_submit(bj_server* s, int32_t rel, const uint32_t* s_key, const uin
                      uint64_t* id)
 {
     if (!s || !id || n_s < 0 || (n_s > 0 && (!s_key || !s_val))) return fail(BJ_ERR_INVALID, "bad argument");
-    if (rel < 0 || rel >= (int32_t)s->rels.size()) return fail(BJ_ERR_INVALID, "no such relation");
+    const Relation* R = find_rel(s, rel);
+    if (!R) return fail(BJ_ERR_INVALID, "no such relation");
     Request* q = new Request();
-    int rc = make_request_plan(s, *s->rels[rel], (uint64_t)n_s, q->pl);
+    int rc = make_request_plan(s, *R, (uint64_t)n_s, q->pl);
     if (rc == BJ_OK) rc = request_items(q->pl, q->items, q->ws_bytes);
     if (rc) {
         delete q;
@@ -517,6 +529,7 @@ int bj_server_submit(bj_server* s, int32_t rel, const uint32_t* s_key, const uin
         return fail(BJ_ERR_SIZE, "request footprint exceeds the server arena");
     }
     q->rel = rel;
+    q->relp = R;
     q->sk = s_key;
     q->sv = s_val;
     {
@@ -557,10 +570,16 @@ int bj_server_wait(bj_server* s, uint64_t id, bj_request_result* out)
 {
     if (!s || !out) return fail(BJ_ERR_INVALID, "bad argument");
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return fail(BJ_ERR_INVALID, "no such request");
-    Request* q = it->second;
-    s->cv_done.wait(lk, [&] { return q->done || s->blocked; });
+    if (s->reqs.find(id) == s->reqs.end()) return fail(BJ_ERR_INVALID, "no such request");
+    // the request is looked up again after every wake-up: another thread may
+    // have released it meanwhile (release needs it done, so it is then gone)
+    Request* q = nullptr;
+    s->cv_done.wait(lk, [&] {
+        auto it = s->reqs.find(id);
+        q = it == s->reqs.end() ? nullptr : it->second;
+        return !q || q->done || s->blocked;
+    });
+    if (!q) return fail(BJ_ERR_INVALID, "request released by another thread while waiting");
     if (!q->done)
         return fail(BJ_ERR_NOMEM, "request queued behind rows that were never released (bj_server_release)");
     q->reported = true;

@@ -22,6 +22,9 @@
  *   bj_gen_build / bj_gen_probe   the dataset generator gen_dataset main()
  *                    (datagen/gen_dataset.cpp:79-263), with the synthetic
  *                    distribution of the benchmark workloads, on the device
+ *   bj_gen_refdist_host / bj_gen_refdist   the same generator with the
+ *                    reference's own distribution (exact files on the host, the
+ *                    same key multiset on the device)
  *   bj_write_output / bj_read_output  output.bin (format defined below; the
  *                    reference defines none)
  */
@@ -209,6 +212,29 @@ int  bj_join(bj_ctx *ctx, const bj_params *p,
              const uint32_t *s_key, const uint32_t *s_val,
              bj_result *out);
 
+/* Size of the device scratch array behind bj_join_noalloc (256 MiB), held on
+ * every GPU the library runs on until the process ends. */
+#define BJ_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+/* No counterpart in the reference.  The join of bj_join (same kernels, same
+ * count and row multiset) for callers that cannot allocate device memory during
+ * a call -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: it calls no hipMalloc / hipFree and creates no events.
+ * Inputs must be device pointers (p->inputs_on_device = 1) joined in one pass
+ * (p->s_chunks = 1), without probe phase timers.  The workspace is carved from a
+ * static device array of BJ_NOALLOC_SCRATCH_BYTES per GPU; a join whose
+ * workspace (bj_plan's workspace_bytes minus the row buffers) exceeds it is
+ * refused with BJ_ERR_SIZE, which at 256 MiB allows about 8-11M tuples per
+ * relation.  Rows go to the caller's device arrays out_key / out_rval / out_sval
+ * (out_cap rows each; the count stays exact past it, as in bj_join) and
+ * out->key / r_val / s_val point there.  Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU race: serialise them. */
+int  bj_join_noalloc(bj_ctx *ctx, const bj_params *p,
+                     const uint32_t *r_key, const uint32_t *r_val,
+                     const uint32_t *s_key, const uint32_t *s_val,
+                     uint32_t *out_key, uint32_t *out_rval, uint32_t *out_sval,
+                     uint64_t out_cap, bj_result *out);
+
 /* Two-level radix partition of one relation (device pointers), with the radix
  * bits bj_join would use for p (r_size = n).  Writes the partitioned tuples to
  * out_key/out_val (n entries each) and the sub-bucket starts to out_start
@@ -235,6 +261,24 @@ int  bj_gen_build(bj_ctx *ctx, uint64_t seed, uint64_t n_r,
 int  bj_gen_probe(bj_ctx *ctx, uint64_t seed_r, uint64_t seed_s, uint64_t n_r,
                   uint64_t n_s, uint64_t m_s, double zipf_s,
                   uint32_t *d_key, uint32_t *d_val);
+
+/* ---- the reference datagen's distribution ---------------------------------
+ * The keys of datagen/gen_dataset.cpp (main, :79-263): k = floor(R * pct / 100)
+ * (at most S) matching keys [0, k), each once in R and once in S; R's other keys
+ * k, k+1, ... (below 2^31 - 1); S's other keys R, R+1, ... (below 2^31 - 1).
+ * max_pct >= 0 is the reference's --max load imbalance (the first quarter of S
+ * holds matches at max_pct percent); < 0 = off.  *matches (may be NULL) = k, the
+ * join size.
+ * bj_gen_refdist_host: the reference's order exactly (Fisher-Yates over
+ *   std::mt19937_64 with the reference's seeds, gen_dataset.cpp:67-77, 192,
+ *   210-219), so raw files written from these arrays are byte-identical to the
+ *   reference generator's for the same arguments.  Host arrays; sequential.
+ * bj_gen_refdist: the same key multiset per region in a keyed random order,
+ *   generated on the device (device pointers). */
+int  bj_gen_refdist_host(int64_t r_size, int64_t s_size, double answer_pct, uint64_t seed,
+                         double max_pct, uint32_t *r_key, uint32_t *s_key, int64_t *matches);
+int  bj_gen_refdist(bj_ctx *ctx, int64_t r_size, int64_t s_size, double answer_pct, uint64_t seed,
+                    double max_pct, uint32_t *d_r_key, uint32_t *d_s_key, int64_t *matches);
 
 /* ---- output.bin ----------------------------------------------------------
  * Little-endian: char magic[8] = "BJOUT001", uint64 count, then uint32
@@ -291,8 +335,10 @@ typedef struct bj_main_args {
     /* additions of this library */
     int32_t radix_bits;            /* --radix_bits (total), -1 = auto            */
     double  bf_bits_per_key;       /* --bf_bits_per_key, <= 0 = formula          */
-    int32_t val_mode;              /* --val_mode: 0 = row index (default),
-                                      1 = the reference's i+1 / i+100000001      */
+    int32_t val_mode;              /* --val_mode, without --read_val: 1 = the
+                                      reference's synthesized values, R i+1 and
+                                      S i+100000001 (default, src/hash_kernel.cu:
+                                      330-336); 0 = the bare row index i          */
     int32_t device;                /* --device                                   */
     int32_t verbose;               /* print the reference-style summary          */
 } bj_main_args;

@@ -8,7 +8,7 @@ Each fixture holds inputs and expected outputs:
   ref_count                    the match count printed by the reference's own
                                run_sanity_check (src/sanity_check.cpp), compiled
                                from its source into oracle/_ref by `make -C oracle ref`
-The refgen_* fixture uses the reference's own dataset generator
+The refgen_* fixtures use the reference's own dataset generator
 (datagen/gen_dataset.cpp, compiled into oracle/_ref/gen_dataset) and keeps its raw
 int32 files as they are written (reference input format).
 """
@@ -75,7 +75,21 @@ def main():
         rk = np.fromfile(os.path.join(d, "gen_R2000_ans10.0pct.bin"), np.uint32)
         sk = np.fromfile(os.path.join(d, "gen_S2000_ans10.0pct.bin"), np.uint32)
     save("refgen_2000", rk, np.arange(len(rk)), sk, np.arange(len(sk)))
+    refgen_max()
+
+
+def refgen_max():
+    """The reference generator with its --max load imbalance (hot first quarter of S)."""
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([O.REF_GEN_PATH, "3000", "5000", "7.5", d, "9", "--max", "15"], check=True,
+                       stdout=subprocess.DEVNULL)
+        rk = np.fromfile(os.path.join(d, "gen_R3000_ans7.5_max15.0pct.bin"), np.uint32)
+        sk = np.fromfile(os.path.join(d, "gen_S5000_ans7.5_max15.0pct.bin"), np.uint32)
+    save("refgen_max_3000", rk, np.arange(len(rk)), sk, np.arange(len(sk)))
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["refgen_max"]:
+        refgen_max()
+    else:
+        main()

@@ -211,11 +211,20 @@ def test_gpu_main_reference_files(tmp_path):
     rpath, spath, opath = tmp_path / "R.bin", tmp_path / "S.bin", tmp_path / "output.bin"
     z["r_key"].tofile(rpath)
     z["s_key"].tofile(spath)
+    # without --read_val the reference synthesizes R's values as i + 1 and S's as
+    # i + 100000001 (src/hash_kernel.cu:330-336); the fixture's rows carry row indices
     args = bj.main_args(r_path=str(rpath), s_path=str(spath), out_path=str(opath),
                         r_size=len(z["r_key"]), s_size=len(z["s_key"]), sanity_check=1,
                         use_bf=14, verbose=0)
     res = bj.gpu_main(args)
     assert res.match_count == int(z["count"]) == int(z["ref_count"])
+    k, a, b = bj.read_output(str(opath))
+    expect = z["rows"] + np.array([0, 1, 100000001], np.uint32)
+    assert np.array_equal(np.stack([k, a, b], 1), expect)
+    # --val_mode 0: the bare row index
+    args = bj.main_args(r_path=str(rpath), s_path=str(spath), out_path=str(opath),
+                        r_size=len(z["r_key"]), s_size=len(z["s_key"]), use_bf=14, val_mode=0, verbose=0)
+    bj.gpu_main(args)
     k, a, b = bj.read_output(str(opath))
     assert np.array_equal(np.stack([k, a, b], 1), z["rows"])
     # --read_val: the value block follows the key block

@@ -92,9 +92,10 @@ def test_more_chunks_than_tuples(gpu_ctx):
         assert rows.tolist() == [[9, 2, 10], [0x80000001, 3, 12]]
 
 
-def test_context_after_rejected_call(gpu_ctx):
-    """An error return drains the context's streams; the next chunked join on
-    the same context, from host buffers, is exact."""
+def test_context_reuse_after_rejected_call(gpu_ctx):
+    """A call rejected by argument validation (s_chunks < 0, refused before any
+    copy or kernel is queued) leaves the context usable: the next chunked join on
+    it, from host buffers, is exact."""
     rk = np.array([5, 7, 9, 0x80000001], np.uint32)
     rv = np.arange(4, dtype=np.uint32)
     sk = np.array([9, 1, 0x80000001, 7], np.uint32)
