@@ -194,12 +194,14 @@ int make_plan(const bj_params& p, Plan& pl)
             if (!(fp > 0.0 && fp < 1.0)) return fail(BJ_ERR_INVALID, "true_false_ratio must be in (0,1)");
             num_bits = -R * std::log(fp) / (std::log(2.0) * std::log(2.0));
         }
+        // the derived counts are clamped to [1, 8] as doubles, before the
+        // conversion to int (a filter of many bits per key would overflow it)
         if (p.num_hashes >= 0) {
             k = p.num_hashes;
         } else if (p.num_bits < 0 && p.bf_bits_per_key > 0) {
-            k = (int)std::lround(p.bf_bits_per_key * std::log(2.0));
+            k = (int)std::min(8.0, std::round(p.bf_bits_per_key * std::log(2.0)));
         } else {
-            k = (int)(num_bits / R * std::log(2.0));
+            k = (int)std::min(8.0, std::max(0.0, num_bits / R * std::log(2.0)));
         }
         if (k < 1) k = 1;
         if (p.num_hashes > 8) return fail(BJ_ERR_INVALID, "num_hashes must be in [1, 8]");

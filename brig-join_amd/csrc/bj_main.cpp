@@ -92,6 +92,15 @@ int bj_read_output(const char* path, uint64_t* n, uint32_t** key, uint32_t** r_v
         std::fclose(f);
         return BJ_ERR_IO;
     }
+    // the rest of the file must be exactly the count's three arrays (a damaged
+    // count is refused before anything is allocated for it)
+    long here = std::ftell(f), end = -1;
+    if (here >= 0 && std::fseek(f, 0, SEEK_END) == 0) end = std::ftell(f);
+    if (end < here || std::fseek(f, here, SEEK_SET) != 0 || cnt > (uint64_t)(end - here) / 12 ||
+        cnt * 12 != (uint64_t)(end - here)) {
+        std::fclose(f);
+        return BJ_ERR_IO;
+    }
     uint32_t* a = (uint32_t*)std::malloc(std::max<uint64_t>(cnt, 1) * 4);
     uint32_t* b = (uint32_t*)std::malloc(std::max<uint64_t>(cnt, 1) * 4);
     uint32_t* c = (uint32_t*)std::malloc(std::max<uint64_t>(cnt, 1) * 4);

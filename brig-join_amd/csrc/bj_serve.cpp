@@ -10,6 +10,7 @@
 // fires.  No hipMalloc / hipFree happens while requests run.  The reference
 // has no serving form.
 #include "bj_api.h"
+#include "bj_arena.h"
 #include "bj_internal.h"
 #include "bj_kernels.h"
 
@@ -35,58 +36,6 @@ constexpr int SRV_MAX_SLOTS = 8;
 constexpr uint64_t ALIGN = 256;
 
 uint64_t align_up(uint64_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
-
-// First-fit allocator over [0, size) of the arena, coalescing on free.
-class ArenaAlloc {
-public:
-    void reset(uint64_t size)
-    {
-        free_.clear();
-        if (size) free_[0] = size;
-        size_ = size;
-        used_ = peak_ = 0;
-    }
-    bool alloc(uint64_t bytes, uint64_t& off)
-    {
-        bytes = align_up(std::max<uint64_t>(bytes, 1));
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= bytes) {
-                off = it->first;
-                const uint64_t rest = it->second - bytes;
-                free_.erase(it);
-                if (rest) free_[off + bytes] = rest;
-                used_ += bytes;
-                peak_ = std::max(peak_, used_);
-                return true;
-            }
-        }
-        return false;
-    }
-    void release(uint64_t off, uint64_t bytes)
-    {
-        bytes = align_up(std::max<uint64_t>(bytes, 1));
-        used_ -= bytes;
-        auto it = free_.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-    uint64_t size() const { return size_; }
-    uint64_t peak() const { return peak_; }
-
-private:
-    std::map<uint64_t, uint64_t> free_;
-    uint64_t size_ = 0, used_ = 0, peak_ = 0;
-};
 
 struct Relation {
     uint64_t nr = 0;
