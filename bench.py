@@ -416,6 +416,8 @@ def main() -> int:
                     help="wall-clock budget of the timed CPU count-only runs")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve8: length of the arrival process")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--probe-breakdown", action="store_true",
+                    help="one more join with the probe's in-kernel phase timers (probe_phases_ms)")
     ap.add_argument("--bf-stage", type=int, default=None, help="override: 1 before, 2 after pass 1")
     ap.add_argument("--pass1-bits", type=int, default=None, help="override the first scatter pass bits")
     ap.add_argument("--s-pass1-bits", type=int, default=None, help="override S's first scatter pass bits")
@@ -426,6 +428,8 @@ def main() -> int:
                          "PCIe-inclusive rate, not the HBM-resident one)")
     ap.add_argument("--s-chunks", type=int, default=None,
                     help="join S in this many chunks (host chunks copied while the previous one is joined)")
+    ap.add_argument("--chunk-probe", type=int, default=None,
+                    help="with --s-chunks: 0 one probe after the last chunk (pooled S'), 1 one per chunk")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE) and exit")
@@ -478,6 +482,8 @@ def main() -> int:
         params = dict(params, tile=args.tile)
     if args.s_chunks is not None:
         params = dict(params, s_chunks=args.s_chunks)
+    if args.chunk_probe is not None:
+        params = dict(params, chunk_probe=args.chunk_probe)
     if args.host_inputs:
         # pinned host copies of the generated relations (the device ones are freed)
         def pinned(t):
@@ -519,6 +525,14 @@ def main() -> int:
     if res.match_count != m_s:
         print(f"wrong match count {res.match_count} != {m_s}", file=sys.stderr)
         return 1
+
+    probe_phases = None
+    if args.probe_breakdown:
+        p.probe_breakdown = 1
+        res = ctx.join(p, rk, rv, sk, sv)
+        probe_phases = {k: round(v, 4) for k, v in res.probe_phases().items()}
+        probe_phases["probe_ms"] = round(res.kernels()["probe"], 4)
+        p.probe_breakdown = 0
 
     ms_step = wall / args.steps * 1e3
     value = (n_r + n_s) / (wall / args.steps)
@@ -568,6 +582,8 @@ def main() -> int:
                        "h2d_ms_before_join": round(h2d_sum / args.steps, 4)},
         "kernels": per_kernel,
     }
+    if probe_phases is not None:
+        out["probe_phases_ms"] = probe_phases
     ctx.close()
     host = None
     if not args.no_cpu_baseline:

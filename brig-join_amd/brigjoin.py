@@ -63,6 +63,7 @@ class BjParams(ctypes.Structure):
         ("s_chunks", ctypes.c_int32), ("bf_stage", ctypes.c_int32),
         ("pass1_bits", ctypes.c_int32), ("s_pass1_bits", ctypes.c_int32),
         ("scatter_wg", ctypes.c_int32), ("probe_breakdown", ctypes.c_int32),
+        ("chunk_probe", ctypes.c_int32),
     ]
 
 

@@ -475,9 +475,14 @@ int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* 
     return BJ_OK;
 }
 
-int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
-                    uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, uint64_t cap)
+int enqueue_s(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
+              const uint32_t** s_cnt_out, uint32_t* o_key, uint32_t* o_val, uint64_t o_cap)
 {
+    if (!o_key) {
+        o_key = P<uint32_t>(c->s_key);
+        o_val = P<uint32_t>(c->s_val);
+        o_cap = c->s_key.bytes / 4;
+    }
     hipStream_t st = c->stream;
     uint64_t* ctr = P<uint64_t>(c->counters);
     auto beg = [&](int slot) { return tl_begin(c, slot); };
@@ -494,8 +499,7 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
             BJ_HIP(end(BJ_K_BF_FILTER));
             if ((rc = run_partition(c, pl, pl.p1_s, P<uint32_t>(c->sp_key), P<uint32_t>(c->sp_val), pl.ns,
                                     ctr + CTR_SPRIME,
-                                    pl.G_s, pl.solo_s, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), c->s_key.bytes / 4,
-                                    P<uint64_t>(c->s_start), BJ_K_HIST_S)))
+                                    pl.G_s, pl.solo_s, o_key, o_val, o_cap, P<uint64_t>(c->s_start), BJ_K_HIST_S)))
                 return rc;
         } else {
             // first-level pass of S, then filter each bucket against its L2-resident
@@ -507,32 +511,51 @@ int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s
             BJ_HIP(beg(BJ_K_BF_FILTER));
             BJ_HIP(launch_zero_u32(P<uint32_t>(c->s_fill), 1ull << pl.bits, st));
             BJ_HIP(launch_filter_l1(P<uint32_t>(c->tmp_key), P<uint32_t>(c->tmp_val), P<uint64_t>(c->s_start),
-                                    pl.bits, pl.bits - pl.p1_s, R.filter, pl.geom, P<uint32_t>(c->s_key),
-                                    P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint32_t>(c->s_fill),
+                                    pl.bits, pl.bits - pl.p1_s, R.filter, pl.geom, o_key, o_val, o_cap,
+                                    P<uint32_t>(c->s_fill),
                                     ctr + CTR_SPRIME, ctr + CTR_XCD, ctr + CTR_ERR, c->num_cu, st));
             s_cnt = P<uint32_t>(c->s_fill);
             BJ_HIP(end(BJ_K_BF_FILTER));
         }
     } else {
-        if ((rc = run_partition(c, pl, pl.p1_s, s_key, s_val, pl.ns, nullptr, pl.G_s, pl.solo_s, P<uint32_t>(c->s_key),
-                                P<uint32_t>(c->s_val), c->s_key.bytes / 4, P<uint64_t>(c->s_start),
-                                BJ_K_HIST_S)))
+        if ((rc = run_partition(c, pl, pl.p1_s, s_key, s_val, pl.ns, nullptr, pl.G_s, pl.solo_s, o_key, o_val, o_cap,
+                                P<uint64_t>(c->s_start), BJ_K_HIST_S)))
             return rc;
     }
+    *s_cnt_out = s_cnt;
+    return BJ_OK;
+}
+
+int enqueue_plan_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* sk, const uint32_t* sv,
+                       const uint64_t* s_start, const uint32_t* s_cnt, uint32_t* o_key, uint32_t* o_rval,
+                       uint32_t* o_sval, uint64_t cap)
+{
+    hipStream_t st = c->stream;
+    uint64_t* ctr = P<uint64_t>(c->counters);
     const int nbins = 1 << pl.bits;
     uint64_t* item_start = P<uint64_t>(c->item_start);
-    BJ_HIP(beg(BJ_K_PLAN));
-    BJ_HIP(launch_plan(R.start, P<uint64_t>(c->s_start), s_cnt, pl.bits, (uint32_t)pl.tile, pl.slice,
+    BJ_HIP(tl_begin(c, BJ_K_PLAN));
+    BJ_HIP(launch_plan(R.start, s_start, s_cnt, pl.bits, (uint32_t)pl.tile, pl.slice,
                        pl.max_slices, P<uint32_t>(c->plan_cnt), item_start, ctr + CTR_ITEMS,
                        P<HeavyItem>(c->items), pl.item_cap, ctr + CTR_ERR, st));
-    BJ_HIP(end(BJ_K_PLAN));
-    BJ_HIP(beg(BJ_K_PROBE));
-    BJ_HIP(launch_probe(pl.tile, R.key, R.val, R.start, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val),
-                        P<uint64_t>(c->s_start), s_cnt, pl.bits, pl.slice, P<HeavyItem>(c->items),
-                        item_start + nbins, pl.item_cap, ctr + CTR_WORK, ctr + CTR_OUT, cap, o_key, o_rval,
-                        o_sval, c->num_cu, pl.probe_breakdown ? P<uint64_t>(c->ptimers) : nullptr, st));
-    BJ_HIP(end(BJ_K_PROBE));
+    BJ_HIP(tl_end(c, BJ_K_PLAN));
+    BJ_HIP(tl_begin(c, BJ_K_PROBE));
+    BJ_HIP(launch_probe(pl.tile, R.key, R.val, R.start, sk, sv, s_start, s_cnt, pl.bits, pl.slice,
+                        P<HeavyItem>(c->items), item_start + nbins, pl.item_cap, ctr + CTR_WORK, ctr + CTR_OUT,
+                        cap, o_key, o_rval, o_sval, c->num_cu,
+                        pl.probe_breakdown ? P<uint64_t>(c->ptimers) : nullptr, st));
+    BJ_HIP(tl_end(c, BJ_K_PROBE));
     return BJ_OK;
+}
+
+int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
+                    uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, uint64_t cap)
+{
+    const uint32_t* s_cnt = nullptr;
+    int rc = enqueue_s(c, pl, R, s_key, s_val, &s_cnt);
+    if (rc) return rc;
+    return enqueue_plan_probe(c, pl, R, P<uint32_t>(c->s_key), P<uint32_t>(c->s_val), P<uint64_t>(c->s_start),
+                              s_cnt, o_key, o_rval, o_sval, cap);
 }
 
 }  // namespace bj
@@ -636,7 +659,9 @@ int bj_destroy(bj_ctx* c)
                       &c->s_key,  &c->s_val,    &c->tmp_key, &c->tmp_val,  &c->sp_key,  &c->sp_val,
                       &c->filter, &c->part,     &c->totals,  &c->l1h,      &c->l1off,   &c->colsum,  &c->r_start,
                       &c->s_start, &c->s_fill,  &c->partials, &c->items,  &c->counters, &c->out_key,
-                      &c->out_rval, &c->out_sval, &c->dtmp,  &c->plan_cnt, &c->item_start, &c->ptimers};
+                      &c->out_rval, &c->out_sval, &c->dtmp,  &c->plan_cnt, &c->item_start, &c->ptimers,
+                      &c->pool_key, &c->pool_val, &c->pool_cnt, &c->pool_off, &c->fin_key, &c->fin_val,
+                      &c->fin_start};
     for (DevBuf* b : bufs) release(*b);
     for (hipEvent_t ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -785,7 +810,8 @@ static uint64_t held_bytes(bj_ctx* c)
                             &c->filter, &c->part,   &c->totals,  &c->l1h,     &c->l1off,  &c->colsum,
                             &c->r_start, &c->s_start, &c->s_fill, &c->partials, &c->items, &c->counters,
                             &c->out_key, &c->out_rval, &c->out_sval, &c->dtmp, &c->plan_cnt, &c->item_start,
-                            &c->ptimers};
+                            &c->ptimers, &c->pool_key, &c->pool_val, &c->pool_cnt, &c->pool_off, &c->fin_key,
+                            &c->fin_val, &c->fin_start};
     for (const DevBuf* b : bufs) t += b->bytes;
     return t;
 }
@@ -818,9 +844,12 @@ static int auto_s_chunks(bj_ctx* c, const bj_params& p, const Plan& full, int& c
 
 // S in chunks (the reference's s_chunks loop, src/hash_kernel.cu:646-650,
 // 910-931): R is partitioned and its filter built once; chunk i of S is
-// filtered, partitioned with R's radix bits and probed, its rows appended
-// through the one row counter.  Host chunks are copied on a second stream into
-// two device buffers, chunk i + 1 while chunk i is joined.
+// filtered and partitioned with R's radix bits.  Host chunks are copied on a
+// second stream into two device buffers, chunk i + 1 while chunk i is joined.
+// Pooled (chunk_probe 0, when a pool the size of S fits): every chunk's S' is
+// appended to a pool, and after the last chunk the pieces are merged into one
+// S' layout and probed once, so R streams past the tables once.  Otherwise
+// every chunk is probed on its own, its rows appended through the one counter.
 static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int chunks, const uint32_t* r_key,
                          const uint32_t* r_val, const uint32_t* s_key, const uint32_t* s_val, bj_result* out)
 {
@@ -844,6 +873,29 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
         if ((rc = ensure(c->in_rk, full.nr * 4)) || (rc = ensure(c->in_rv, full.nr * 4)) ||
             (rc = ensure(c->in_sk, 2 * chunk * 4)) || (rc = ensure(c->in_sv, 2 * chunk * 4)))
             return rc;
+    }
+    const int nbins = 1 << cp.bits;
+    // the pool holds up to all of S, the merged S' as much again (its size is
+    // known after the last chunk), the piece table a row per chunk
+    bool pooled = false;
+    if (p->chunk_probe == 0) {
+        const uint64_t table = (uint64_t)nch * nbins * 12;
+        const uint64_t need = 2 * ns * 8 + table;
+        const uint64_t held = c->pool_key.bytes + c->pool_val.bytes + c->pool_cnt.bytes + c->pool_off.bytes +
+                              c->fin_key.bytes + c->fin_val.bytes;
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            (void)hipGetLastError();
+        } else if (need <= (uint64_t)((double)(fr + held) * 0.9)) {
+            pooled = true;
+            if ((rc = ensure(c->pool_key, ns * 4)) || (rc = ensure(c->pool_val, ns * 4)) ||
+                (rc = ensure(c->pool_cnt, (uint64_t)nch * nbins * 4)) ||
+                (rc = ensure(c->pool_off, (uint64_t)nch * nbins * 8)) ||
+                (rc = ensure(c->fin_start, ((uint64_t)nbins + 1) * 8)))
+                return rc;
+        }
+    } else if (p->chunk_probe != 1) {
+        return fail(BJ_ERR_INVALID, "chunk_probe must be 0 or 1");
     }
 
     tl_reset(c, timing);
@@ -869,7 +921,6 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
     if ((rc = probe_timers_begin(c, cp))) return rc;
     if ((rc = enqueue_r(c, cp, r_key, r_val))) return rc;
     const RSide R{P<uint32_t>(c->r_key), P<uint32_t>(c->r_val), P<uint64_t>(c->r_start), P<uint64_t>(c->filter)};
-    const int nbins = 1 << cp.bits;
     uint64_t* item_start = P<uint64_t>(c->item_start);
 
     // buffer b of the two holds chunks b, b + 2, ...; its next copy waits until
@@ -887,8 +938,14 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
     };
     static const int s_slots[] = {BJ_K_BF_FILTER, BJ_K_HIST_S,  BJ_K_SCAN_S, BJ_K_SCATTER1_S,
                                   BJ_K_SCATTER2_S, BJ_K_PLAN, BJ_K_PROBE};
-    constexpr size_t NSL = sizeof(s_slots) / sizeof(s_slots[0]);
-    std::vector<int> s_events;  // per chunk and S-side slot: begin, end
+    struct SlotSpan { int slot, b, e; };
+    std::vector<SlotSpan> s_events;  // per chunk and S-side slot: its begin and end events
+    auto take_slots = [&]() {
+        for (int sl : s_slots) {
+            if (c->tl_b[sl] >= 0 && c->tl_e[sl] >= 0) s_events.push_back({sl, c->tl_b[sl], c->tl_e[sl]});
+            c->tl_b[sl] = c->tl_e[sl] = -1;
+        }
+    };
     if (host && (rc = copy_chunk(0))) return rc;
     for (int i = 0; i < nch; i++) {
         const int b = i & 1;
@@ -900,13 +957,32 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
             ck = P<uint32_t>(c->in_sk) + b * chunk;
             cv = P<uint32_t>(c->in_sv) + b * chunk;
         }
-        // per-chunk counters: S', work items, filter tiles (rows and error flags carry over)
+        // per-chunk counters: S', work items, filter tiles (rows, the pool and
+        // error flags carry over)
         BJ_HIP(launch_zero_u64(ctr + CTR_SPRIME, 1, st));
         BJ_HIP(launch_zero_u64(ctr + CTR_WORK, 2, st));
         BJ_HIP(launch_zero_u64(ctr + CTR_XCD, CTR_N - CTR_XCD, st));
         c->tl_open = false;  // the chunk's first slot starts after the wait and the clears
         Plan pi = cp;
         pi.ns = n;
+        if (pooled) {
+            // the chunk's S' stays in its own region of the pool, [off, off + n),
+            // and its pieces (sub-bucket counts and places) go to row i of the
+            // piece table (timed with the second S pass)
+            const uint32_t* s_cnt = nullptr;
+            if ((rc = enqueue_s(c, pi, R, ck, cv, &s_cnt, P<uint32_t>(c->pool_key) + off,
+                                P<uint32_t>(c->pool_val) + off, n)))
+                return rc;
+            if (host) BJ_HIP(hipEventRecord(c->ev_used[b], st));  // the chunk's input is read
+            take_slots();
+            BJ_HIP(tl_begin(c, BJ_K_SCATTER2_S));
+            BJ_HIP(launch_pool_note(P<uint64_t>(c->s_start), s_cnt, nbins, off, P<uint32_t>(c->pool_cnt) + (size_t)i * nbins,
+                                    P<uint64_t>(c->pool_off) + (size_t)i * nbins, ctr + CTR_POOL, st));
+            BJ_HIP(tl_end(c, BJ_K_SCATTER2_S));
+            take_slots();
+            if (host && i + 1 < nch && (rc = copy_chunk(i + 1))) return rc;
+            continue;
+        }
         if ((rc = enqueue_s_probe(c, pi, R, ck, cv, P<uint32_t>(c->out_key), P<uint32_t>(c->out_rval),
                                   P<uint32_t>(c->out_sval), cp.cap)))
             return rc;
@@ -917,27 +993,52 @@ static int join_s_chunks(bj_ctx* c, const bj_params* p, const Plan& full, int ch
             BJ_HIP(hipEventRecord(c->ev_used[b], st));
             if (i + 1 < nch && (rc = copy_chunk(i + 1))) return rc;
         }
-        if (timing) {
-            for (int sl : s_slots) {
-                s_events.push_back(c->tl_b[sl]);
-                s_events.push_back(c->tl_e[sl]);
-                c->tl_b[sl] = c->tl_e[sl] = -1;
-            }
-        }
+        take_slots();
+    }
+    int last_end = s_events.empty() ? -1 : s_events.back().e;
+    if (pooled) {
+        // merge the pieces into one S' layout (its size read back first), then
+        // one build + probe
+        BJ_HIP(tl_begin(c, BJ_K_SCATTER2_S));
+        BJ_HIP(launch_pool_start(P<uint32_t>(c->pool_cnt), nch, nbins, P<uint32_t>(c->totals),
+                                 P<uint64_t>(c->fin_start), P<uint64_t>(c->partials), st));
+        BJ_HIP(hipMemcpyAsync(c->h_counters + CTR_N, P<uint64_t>(c->fin_start) + nbins, sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, st));
+        BJ_HIP(hipStreamSynchronize(st));
+        const uint64_t total = c->h_counters[CTR_N];
+        if (total > ns) return fail(BJ_ERR_INTERNAL, "pooled S' larger than S");
+        if ((rc = ensure(c->fin_key, total * 4)) || (rc = ensure(c->fin_val, total * 4))) return rc;
+        BJ_HIP(launch_pool_merge(P<uint32_t>(c->pool_key), P<uint32_t>(c->pool_val), P<uint32_t>(c->pool_cnt),
+                                 P<uint64_t>(c->pool_off), nch, nbins, P<uint64_t>(c->fin_start),
+                                 P<uint32_t>(c->fin_key), P<uint32_t>(c->fin_val), c->fin_key.bytes / 4,
+                                 ctr + CTR_ERR, st));
+        BJ_HIP(tl_end(c, BJ_K_SCATTER2_S));
+        take_slots();
+        BJ_HIP(launch_zero_u64(ctr + CTR_WORK, 2, st));
+        Plan pf = cp;
+        pf.ns = total;
+        c->tl_open = false;
+        if ((rc = enqueue_plan_probe(c, pf, R, P<uint32_t>(c->fin_key), P<uint32_t>(c->fin_val),
+                                     P<uint64_t>(c->fin_start), nullptr, P<uint32_t>(c->out_key),
+                                     P<uint32_t>(c->out_rval), P<uint32_t>(c->out_sval), cp.cap)))
+            return rc;
+        last_end = c->tl_e[BJ_K_PROBE];
     }
     BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     BJ_HIP(hipStreamSynchronize(st));
 
     const uint64_t* h = c->h_counters;
     if (h[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(h[CTR_ERR]));
-    fill_result(c, cp, h, full.bf ? h[CTR_ACC_SPRIME] : ns, h[CTR_ACC_ITEMS], out);
+    if (pooled)
+        fill_result(c, cp, h, h[CTR_POOL], h[CTR_ITEMS], out);
+    else
+        fill_result(c, cp, h, full.bf ? h[CTR_ACC_SPRIME] : ns, h[CTR_ACC_ITEMS], out);
     out->s_chunks = nch;
     if (timing) {
         double* k = out->kernel_ms;
         for (int i = 0; i < BJ_NUM_KERNELS; i++) k[i] = tl_ms(c, i);
-        for (size_t e = 0; e < s_events.size(); e += 2) k[s_slots[(e / 2) % NSL]] += ev_ms(c, s_events[e], s_events[e + 1]);
-        // the span ends at the last chunk's probe (its end event is the last of the list)
-        const int last_end = s_events.empty() ? -1 : s_events.back();
+        for (const SlotSpan& e : s_events) k[e.slot] += ev_ms(c, e.b, e.e);
+        // the span ends at the last probe
         fill_phases(out, h2d_ev ? ev_ms(c, EV_H2D0, EV_H2D1) : 0.0, ev_ms(c, c->tl_b[BJ_K_HIST_R], last_end));
     }
     return probe_timers_end(c, cp, out);

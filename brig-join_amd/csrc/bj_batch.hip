@@ -413,7 +413,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             sb = sb_lo + (uint32_t)(item - n_heavy);
         }
         const uint64_t r0 = r_start[sb], nr = r_cnt[sb], s0 = s_start[sb], ns = s_cnt[sb];
-        const bool build_r = nr <= ns;
+        const bool build_r = build_from_r(nr, ns, (uint32_t)CAP);
         const uint64_t nb = build_r ? nr : ns, np = build_r ? ns : nr;
         const bool skip = nr == 0 || ns == 0 || (!heavy && (nb > (uint64_t)CAP || np > (uint64_t)slice));
         if (!skip) {

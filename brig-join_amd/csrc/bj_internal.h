@@ -49,7 +49,9 @@ int ceil_log2(uint64_t n);
 // CTR_ITEMS: probe work items of the last plan (written by the plan kernel);
 // CTR_XCD .. CTR_XCD + 7: tile counters of the filter kernel, one per XCD;
 // CTR_ACC_SPRIME / CTR_ACC_ITEMS: |S'| and vlsplit items summed over the chunks of S
-enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_ACC_SPRIME, CTR_ACC_ITEMS, CTR_XCD = 8, CTR_N = 16 };
+// CTR_POOL: tuples in the pool of S' (S in chunks with one probe)
+enum { CTR_SPRIME = 0, CTR_OUT, CTR_WORK, CTR_ITEMS, CTR_ERR, CTR_ACC_SPRIME, CTR_ACC_ITEMS, CTR_POOL, CTR_XCD = 8,
+       CTR_N = 16 };
 
 // Events EV_H2D0/1 bracket the host copies; EV_KB .. are a pool for the
 // kernel-slot boundaries (tl_begin / tl_end), EV_N at creation, grown when a
@@ -112,6 +114,10 @@ struct bj_ctx {
     bj::DevBuf plan_cnt, item_start;
     bj::DevBuf out_key, out_rval, out_sval, dtmp;
     bj::DevBuf ptimers;  // probe phase timers (probe_breakdown)
+    // S in chunks with one probe: every chunk's S' (pool_key / pool_val), the
+    // piece table (pool_cnt / pool_off, one row per chunk) and the merged S'
+    // (fin_key / fin_val at fin_start)
+    bj::DevBuf pool_key, pool_val, pool_cnt, pool_off, fin_key, fin_val, fin_start;
     uint64_t* h_counters = nullptr;  // pinned
     // S in chunks: copies of host inputs run on copy_stream, R's first (then
     // ev_r_copied), then the chunks of S into two device buffers (in_sk / in_sv,
@@ -134,7 +140,18 @@ int ensure_workspace(bj_ctx* c, const Plan& pl, uint64_t* total_bytes = nullptr)
 // R partition (and the filter built from it) into c's r_key / r_val / r_start / filter.
 int enqueue_r(bj_ctx* c, const Plan& pl, const uint32_t* r_key, const uint32_t* r_val);
 // S (filtered against R's filter when the plan has one) partitioned with R's
-// radix bits, then the build + probe against R; rows to o_* (capacity cap).
+// radix bits into o_key / o_val (capacity o_cap; null: c's s_key / s_val) at
+// the sub-bucket starts c->s_start; *s_cnt is the survivors per sub-bucket
+// where the regions are not full (filter after the first pass, c->s_fill),
+// else null.
+int enqueue_s(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
+              const uint32_t** s_cnt, uint32_t* o_key = nullptr, uint32_t* o_val = nullptr, uint64_t o_cap = 0);
+// Work items and the build + probe of a partitioned S (sk / sv at s_start, with
+// s_cnt tuples per sub-bucket when not null) against R; rows to o_* (capacity cap).
+int enqueue_plan_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* sk, const uint32_t* sv,
+                       const uint64_t* s_start, const uint32_t* s_cnt, uint32_t* o_key, uint32_t* o_rval,
+                       uint32_t* o_sval, uint64_t cap);
+// enqueue_s, then enqueue_plan_probe on its output.
 int enqueue_s_probe(bj_ctx* c, const Plan& pl, const RSide& R, const uint32_t* s_key, const uint32_t* s_val,
                     uint32_t* o_key, uint32_t* o_rval, uint32_t* o_sval, uint64_t cap);
 // One-pass join of device-resident inputs into o_* (capacity pl.cap), with the

@@ -71,6 +71,21 @@ hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const ui
                             int bits2, const uint64_t* filter, const BloomGeom& g, uint32_t* okey,
                             uint32_t* oval, uint64_t out_cap, uint32_t* s_fill, uint64_t* count,
                             uint64_t* tile_ctr, uint64_t* err, int num_cu, hipStream_t st);
+// S in chunks, one probe after the last chunk: each chunk's S' stays in its own
+// region of a pool (starting at base); launch_pool_note records its pieces
+// (sub-bucket sb: s_cnt[sb] tuples when not null, else the region's length, at
+// base + s_start[sb]) in this chunk's row cnt / off of the piece table (nbins
+// entries each) and adds them to *total.  launch_pool_start sums the table over
+// nch chunks into totals and scans them into start (nbins + 1 entries);
+// launch_pool_merge gathers every sub-bucket's pieces into
+// [start[sb], start[sb + 1]) of ok / ov (err |= 16 past out_cap).
+hipError_t launch_pool_note(const uint64_t* s_start, const uint32_t* s_cnt, int nbins, uint64_t base, uint32_t* cnt,
+                            uint64_t* off, uint64_t* total, hipStream_t st);
+hipError_t launch_pool_start(const uint32_t* cnt, int nch, int nbins, uint32_t* totals, uint64_t* start,
+                             uint64_t* partials, hipStream_t st);
+hipError_t launch_pool_merge(const uint32_t* pk, const uint32_t* pv, const uint32_t* cnt, const uint64_t* off,
+                             int nch, int nbins, const uint64_t* start, uint32_t* ok, uint32_t* ov,
+                             uint64_t out_cap, uint64_t* err, hipStream_t st);
 hipError_t launch_plan(const uint64_t* r_start, const uint64_t* s_start, const uint32_t* s_cnt, int bits,
                        uint32_t tile, uint32_t slice, uint32_t max_slices, uint32_t* counts,
                        uint64_t* item_start, uint64_t* total_out, HeavyItem* items, uint64_t item_cap,

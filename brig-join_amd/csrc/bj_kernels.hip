@@ -7,7 +7,7 @@
 //       bucket, XCD-aware, filter slice L2-resident) places the survivors;
 //       bf_stage 1: k_bloom_prefilter (S -> S') then the two-level partition
 //   J:  k_plan_count, k_plan_fill (vlsplit work items) -> k_probe (persistent;
-//       LDS chained hash table on the smaller side, stream the other side)
+//       LDS chained hash table on the build side, stream the other side)
 //
 // Reference counterparts (Chanho-LEE-SNU/BRIG-Join):
 //   k_hist / k_colsums / k_starts    gpu_run_hist_for_both, gpu_make_simple_prefix,
@@ -721,9 +721,70 @@ __global__ __launch_bounds__(BLOCK) void k_filter_l1(
     if (bad) atomicOr((unsigned long long*)err, 8ull);
 }
 
+// ============================================================== S' of the chunks of S
+// With S in chunks, every chunk's S' (its surviving tuples, in sub-bucket
+// regions) stays in its own region of a pool, and one probe runs after the
+// last chunk on all of them, so R streams past the tables once instead of once
+// per chunk.  Note: sub-bucket sb of the chunk whose region starts at base has
+// its piece at base + s_start[sb], cnt[sb] tuples long (the chunk's row of the
+// piece table); *total accumulates the tuples.
+constexpr int POOL_BLOCK = 256;
+
+__global__ __launch_bounds__(POOL_BLOCK) void k_pool_note(const uint64_t* __restrict__ s_start,
+                                                          const uint32_t* __restrict__ s_cnt, int nbins, uint64_t base,
+                                                          uint32_t* __restrict__ cnt, uint64_t* __restrict__ off,
+                                                          uint64_t* __restrict__ total)
+{
+    const int sb = blockIdx.x * POOL_BLOCK + threadIdx.x;
+    uint64_t n = 0;
+    if (sb < nbins) {
+        const uint64_t s0 = s_start[sb];
+        n = s_cnt ? (uint64_t)s_cnt[sb] : s_start[sb + 1] - s0;
+        cnt[sb] = (uint32_t)n;
+        off[sb] = base + s0;
+    }
+    n = wave_sum<uint64_t>(n);
+    if (lane_id() == 0 && n) atomicAdd((unsigned long long*)total, (unsigned long long)n);
+}
+
+// totals[sb] = the pieces of sub-bucket sb summed over the nch chunks.
+__global__ void k_pool_totals(const uint32_t* __restrict__ cnt, int nch, int nbins, uint32_t* __restrict__ totals)
+{
+    const int sb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= nbins) return;
+    uint32_t t = 0;
+    for (int c = 0; c < nch; c++) t += cnt[(size_t)c * nbins + sb];
+    totals[sb] = t;
+}
+
+// Sub-bucket sb (one workgroup) gathers its pieces, chunk by chunk, into
+// [start[sb], start[sb + 1]) of ok / ov.
+__global__ __launch_bounds__(POOL_BLOCK) void k_pool_merge(
+    const uint32_t* __restrict__ pk, const uint32_t* __restrict__ pv, const uint32_t* __restrict__ cnt,
+    const uint64_t* __restrict__ off, int nch, int nbins, const uint64_t* __restrict__ start,
+    uint32_t* __restrict__ ok, uint32_t* __restrict__ ov, uint64_t out_cap, uint64_t* __restrict__ err)
+{
+    const int sb = blockIdx.x;
+    uint64_t d = start[sb];
+    if (start[sb + 1] > out_cap) {
+        if (threadIdx.x == 0) atomicOr((unsigned long long*)err, 16ull);
+        return;
+    }
+    for (int c = 0; c < nch; c++) {
+        const uint32_t n = cnt[(size_t)c * nbins + sb];
+        const uint64_t o = off[(size_t)c * nbins + sb];
+        for (uint32_t i = threadIdx.x; i < n; i += POOL_BLOCK) {
+            ok[d + i] = pk[o + i];
+            ov[d + i] = pv[o + i];
+        }
+        d += n;
+    }
+}
+
 // ============================================================== probe planning
-// A sub-bucket whose smaller side fits one LDS tile and whose larger side fits
-// one slice is "regular": one work item, indexed by the sub-bucket itself.
+// A sub-bucket whose build side (build_from_r: R when it fits one LDS tile,
+// else the smaller side) fits one LDS tile and whose probe side fits one slice
+// is "regular": one work item, indexed by the sub-bucket itself.
 // Larger ones are split (vlsplit): the build side into ceil(nb / tile) chunks,
 // the probe side into min(ceil(np / slice), max_slices) slices, one explicit
 // work item per (chunk, slice).  Count, scan, fill: no contention, and the
@@ -840,7 +901,7 @@ __global__ __launch_bounds__(1024) void k_plan_fill(const uint64_t* __restrict__
 // Persistent workgroups pull work items from a global counter (the next item
 // is fetched while the current one runs): first the vlsplit items (the
 // largest), then one item per regular sub-bucket.  Each item builds the
-// smaller side's chunk into an LDS table and streams the other side's slice
+// build side's chunk into an LDS table and streams the other side's slice
 // past it (probe_item, bj_tiles.h).
 // COLLECT (the reference's COLLECT / --fused_probe_breakdown): thread 0 of
 // every workgroup adds its phase times (PT_*) to timers[blockIdx.x * PT_N + p].
@@ -891,7 +952,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         }
         uint64_t r0, nr, s0, ns;
         sb_sizes(r_start, s_start, s_cnt, (int)sb, r0, nr, s0, ns);
-        const bool build_r = nr <= ns;
+        const bool build_r = build_from_r(nr, ns, (uint32_t)CAP);
         const uint64_t nb = build_r ? nr : ns, np = build_r ? ns : nr;
         // empty, or a regular index whose sub-bucket is on the vlsplit list: nothing to do
         const bool skip = nr == 0 || ns == 0 || (!heavy && (nb > (uint64_t)CAP || np > (uint64_t)slice));
@@ -1158,6 +1219,31 @@ hipError_t launch_filter_l1(const uint32_t* ikey, const uint32_t* ival, const ui
     else if (g.k == 4) BJ_FILTER(4);
     else BJ_FILTER(0);
 #undef BJ_FILTER
+    return hipGetLastError();
+}
+
+hipError_t launch_pool_note(const uint64_t* s_start, const uint32_t* s_cnt, int nbins, uint64_t base, uint32_t* cnt,
+                            uint64_t* off, uint64_t* total, hipStream_t st)
+{
+    k_pool_note<<<ceil_div_i((uint64_t)nbins, POOL_BLOCK), POOL_BLOCK, 0, st>>>(s_start, s_cnt, nbins, base, cnt, off,
+                                                                                total);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool_start(const uint32_t* cnt, int nch, int nbins, uint32_t* totals, uint64_t* start,
+                             uint64_t* partials, hipStream_t st)
+{
+    k_pool_totals<<<ceil_div_i((uint64_t)nbins, 256), 256, 0, st>>>(cnt, nch, nbins, totals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_scan(totals, (uint64_t)nbins, start, partials, st);
+}
+
+hipError_t launch_pool_merge(const uint32_t* pk, const uint32_t* pv, const uint32_t* cnt, const uint64_t* off,
+                             int nch, int nbins, const uint64_t* start, uint32_t* ok, uint32_t* ov,
+                             uint64_t out_cap, uint64_t* err, hipStream_t st)
+{
+    k_pool_merge<<<nbins, POOL_BLOCK, 0, st>>>(pk, pv, cnt, off, nch, nbins, start, ok, ov, out_cap, err);
     return hipGetLastError();
 }
 

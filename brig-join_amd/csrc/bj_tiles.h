@@ -743,17 +743,27 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
     // the caller synchronises before the next item rebuilds the table
 }
 
-// Work-item split of one sub-bucket (vlsplit).  A sub-bucket whose smaller side
-// fits one LDS tile and whose larger side fits one slice is "regular": one work
-// item, indexed by the sub-bucket itself.  Larger ones are split: the build
-// side into ceil(nb / tile) chunks, the probe side into
+// Build side of a sub-bucket: R whenever it fits one LDS table (the
+// reference's build side), else the smaller side.  Equal S keys, as under a
+// skewed S, then stream past distinct R keys instead of piling up in one run
+// of the table that every matching probe walks.
+__host__ __device__ __forceinline__ bool build_from_r(uint64_t nr, uint64_t ns, uint32_t tile)
+{
+    return nr <= tile || nr <= ns;
+}
+
+// Work-item split of one sub-bucket (vlsplit).  A sub-bucket whose build side
+// (build_from_r) fits one LDS tile and whose probe side fits one slice is
+// "regular": one work item, indexed by the sub-bucket itself.  Larger ones are
+// split: the build side into ceil(nb / tile) chunks, the probe side into
 // min(ceil(np / slice), max_slices) slices, one explicit item per (chunk, slice).
 __device__ __forceinline__ uint32_t plan_items(uint64_t nr, uint64_t ns, uint32_t tile, uint32_t slice,
                                                uint32_t max_slices, uint32_t& chunks, uint32_t& slices)
 {
     chunks = slices = 0;
     if (nr == 0 || ns == 0) return 0;
-    const uint64_t nb = min(nr, ns), np = max(nr, ns);
+    const bool br = build_from_r(nr, ns, tile);
+    const uint64_t nb = br ? nr : ns, np = br ? ns : nr;
     if (nb <= tile && np <= slice) return 0;
     chunks = (uint32_t)((nb + tile - 1) / tile);
     slices = (uint32_t)min<uint64_t>((np + slice - 1) / slice, max_slices);

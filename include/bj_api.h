@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define BJ_API_VERSION 3
+#define BJ_API_VERSION 4
 
 /* ---- error codes ------------------------------------------------------- */
 #define BJ_OK              0
@@ -142,6 +142,11 @@ typedef struct bj_params {
     int32_t probe_breakdown;        /* --fused_probe_breakdown: 1 = time the probe's
                                        phases (bj_result.probe_phase_ms; implies
                                        collect_phase_times)                          */
+    int32_t chunk_probe;            /* S in chunks: 0 = one build + probe after the
+                                       last chunk on every chunk's S' (pooled in device
+                                       memory; R streams past the tables once) when a
+                                       pool the size of S fits 90% of the free device
+                                       memory, else one per chunk; 1 = one per chunk */
 } bj_params;
 
 /* ---- join result ------------------------------------------------------- */

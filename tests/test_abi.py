@@ -40,7 +40,7 @@ def test_library_loads_and_exports_every_declared_symbol():
 
 def test_version_and_strerror():
     L = bj.lib()
-    assert L.bj_version() == 3
+    assert L.bj_version() == 4
     assert L.bj_strerror(0) == b"ok"
     for code in range(-7, 0):
         assert L.bj_strerror(code) not in (b"ok", b"unknown error")

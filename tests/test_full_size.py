@@ -40,6 +40,8 @@ def relations(gpu_ctx):
 @pytest.mark.parametrize("kw", [
     dict(partition_size=512, radix_bits=14, use_bf=1, bf_bits_per_key=10.0, num_hashes=3),
     dict(partition_size=512, radix_bits=14, use_bf=1, bf_bits_per_key=10.0, num_hashes=3, s_chunks=4),
+    dict(partition_size=512, radix_bits=14, use_bf=1, bf_bits_per_key=10.0, num_hashes=3, s_chunks=4,
+         chunk_probe=1),
 ])
 def test_full_size_join(gpu_ctx, relations, kw):
     m, dev, rkh, skh = relations

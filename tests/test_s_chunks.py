@@ -6,7 +6,10 @@ same match count and the same sorted multiset of (key, r_val, s_val) rows as
 the oracle's CPU join, for device and host inputs (host chunks are copied on a
 second stream while the previous chunk is joined), chunk counts that do and do
 not divide |S|, more chunks than S tuples, the filter on (both stages) and
-off, duplicate-heavy keys, and a row capacity exceeded across chunks.
+off, duplicate-heavy keys, and a row capacity exceeded across chunks.  Both
+probe modes: every chunk's S' pooled and probed once after the last chunk
+(chunk_probe 0, the default where the pool fits) and one probe per chunk
+(chunk_probe 1).
 """
 import numpy as np
 import pytest
@@ -49,10 +52,15 @@ CASES = [
 ]
 
 
+MODES = pytest.mark.parametrize("cprobe", [0, 1], ids=["pooled", "per_chunk"])
+
+
+@MODES
 @pytest.mark.parametrize("host", [False, True], ids=["device", "host"])
 @pytest.mark.parametrize("ci", range(len(CASES)))
-def test_chunks_match_oracle(gpu_ctx, ci, host):
+def test_chunks_match_oracle(gpu_ctx, ci, host, cprobe):
     n_r, n_s, ratio, kw = CASES[ci]
+    kw = dict(kw, chunk_probe=cprobe)
     m = O.match_count(n_s, ratio)
     rk, rv = O.gen_r(42, n_r)
     sk, sv = O.gen_s(42, 42 ^ 0xDEADBEEF, n_r, n_s, m)
@@ -67,7 +75,8 @@ def test_chunks_match_oracle(gpu_ctx, ci, host):
     assert res.phases()["total"] >= ks["probe"]
 
 
-def test_chunks_same_survivors_as_one_pass(gpu_ctx):
+@MODES
+def test_chunks_same_survivors_as_one_pass(gpu_ctx, cprobe):
     """|S'| is a per-tuple property of the filter: the chunk sums equal one pass."""
     n_r, n_s = 1 << 20, 1 << 21
     m = O.match_count(n_s, 0.1)
@@ -75,7 +84,7 @@ def test_chunks_same_survivors_as_one_pass(gpu_ctx):
     sk, sv = O.gen_s(42, 42 ^ 0xDEADBEEF, n_r, n_s, m)
     kw = dict(use_bf=1, bf_stage=2, radix_bits=12, bf_bits_per_key=8.0, num_hashes=3)
     one, rows1 = run(gpu_ctx, rk, rv, sk, sv, False, s_chunks=1, **kw)
-    many, rows5 = run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=5, **kw)
+    many, rows5 = run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=5, chunk_probe=cprobe, **kw)
     assert one.s_chunks == 1 and many.s_chunks == 5
     assert one.s_survivors == many.s_survivors and one.s_survivors >= m
     assert np.array_equal(rows1, rows5)
@@ -107,7 +116,8 @@ def test_context_reuse_after_rejected_call(gpu_ctx):
     assert rows.tolist() == [[7, 1, 13], [9, 2, 10], [0x80000001, 3, 12]]
 
 
-def test_duplicates_across_chunks(gpu_ctx):
+@MODES
+def test_duplicates_across_chunks(gpu_ctx, cprobe):
     """Skewed S (every key of a small set, many times) with duplicate R keys."""
     rng = np.random.default_rng(7)
     rk = rng.integers(1, 300, 20_000, dtype=np.uint32)
@@ -115,19 +125,21 @@ def test_duplicates_across_chunks(gpu_ctx):
     sk = rng.integers(1, 400, 150_000, dtype=np.uint32)
     sv = np.arange(len(sk), dtype=np.uint32)
     n, k, a, b = O.cpu_join(rk, rv, sk, sv)
-    res, rows = run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=3, answer_scale=float(n) / len(sk) + 1.0)
+    res, rows = run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=3, answer_scale=float(n) / len(sk) + 1.0,
+                    chunk_probe=cprobe)
     assert res.match_count == n and res.truncated == 0
     assert np.array_equal(rows, O.sort_rows(k, a, b))
 
 
-def test_capacity_exceeded_across_chunks(gpu_ctx):
+@MODES
+def test_capacity_exceeded_across_chunks(gpu_ctx, cprobe):
     """The count stays exact and the written rows are valid matches."""
     rk = np.repeat(np.arange(1, 11, dtype=np.uint32), 50)   # 10 keys x 50 rows
     rv = np.arange(len(rk), dtype=np.uint32)
     sk = np.tile(np.arange(1, 21, dtype=np.uint32), 40)     # 800 S tuples, half match 50 each
     sv = np.arange(len(sk), dtype=np.uint32)
     n, _, _, _ = O.cpu_join(rk, rv, sk, sv)
-    res, rows = run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=4)
+    res, rows = run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=4, chunk_probe=cprobe)
     assert res.match_count == n == 400 * 50 and res.truncated == 1
     assert res.rows_written == res.capacity == 800
     assert np.all(rk[rows[:, 1]] == rows[:, 0]) and np.all(sk[rows[:, 2]] == rows[:, 0])
@@ -141,3 +153,9 @@ def test_auto_chunks_fit_in_one_pass(gpu_ctx):
     sk, sv = O.gen_s(42, 42 ^ 0xDEADBEEF, n_r, n_s, m)
     res, rows = run(gpu_ctx, rk, rv, sk, sv, True, s_chunks=0, use_bf=1)
     assert res.s_chunks == 1 and res.match_count == m
+
+
+def test_chunk_probe_invalid(gpu_ctx):
+    rk = np.array([5, 7], np.uint32)
+    with pytest.raises(bj.BjError):
+        run(gpu_ctx, rk, rk, rk, rk, False, s_chunks=2, chunk_probe=2)
