@@ -130,6 +130,7 @@ class BjJoinOut(ctypes.Structure):
         ("capacity", ctypes.c_uint64), ("s_survivors", ctypes.c_uint64),
         ("truncated", ctypes.c_int32), ("radix_bits", ctypes.c_int32),
         ("tile", ctypes.c_int32), ("bf_k", ctypes.c_int32),
+        ("bf_stage", ctypes.c_int32), ("bf_pass_bits", ctypes.c_int32),
         ("bf_bits_per_key", ctypes.c_double),
         ("key", ctypes.c_void_p), ("r_val", ctypes.c_void_p), ("s_val", ctypes.c_void_p),
     ]

@@ -31,11 +31,13 @@ constexpr uint32_t B_MAX_SLICES = 16;
 constexpr uint64_t B_SEG_TARGET = 64 * 1024;     // Bloom segment built in LDS
 constexpr uint64_t B_SEG_MAX = 160 * 1024;
 constexpr int B_MAX_JOINS = 1 << 20;
-// A batch filters S before partitioning it, each key probing a random filter
-// word: fast while the join's filter stays in an XCD's L2, several times slower
-// from the MALL or HBM.  A filter chosen by the tuner or the default rule is
-// kept only up to this size.
+// A filter up to this size stays in an XCD's L2 while S probes it at random,
+// so S is filtered before it is partitioned.  A larger one (bf2) is probed after
+// a first pass of S by the top fb bits, so that a chunk of that pass probes a
+// slice of about B_SLICE_BYTES (the same order as the single join's filter
+// after its first pass).
 constexpr uint64_t B_FILTER_L2_BYTES = 3584ull << 10;
+constexpr uint64_t B_SLICE_BYTES = 128ull << 10;
 
 struct JoinCfg {
     int bits = 1;
@@ -43,6 +45,8 @@ struct JoinCfg {
     int tile = 8192;
     double bpk = 0.0;
     int k = 0;
+    bool bf2 = false;  // filter after a first pass of S by fb bits
+    int fb = 0;
     BloomGeom geom{0, 1, 1};
     uint64_t cap = 1;
 };
@@ -71,6 +75,10 @@ int sr_class(uint64_t nr, uint64_t ns)
     return std::max(-3, std::min(3, (int)std::lround(l)));
 }
 
+// Filter class of a join with |R| = nr: 1 when a 10 bits/key filter exceeds
+// what stays in the L2 (it would be probed after a first pass of S), else 0.
+int filter_class(uint64_t nr) { return (double)nr * 10.0 / 8.0 > (double)B_FILTER_L2_BYTES ? 1 : 0; }
+
 int rule_bits(uint64_t m, int tile)
 {
     const double per = 0.6 * tile;
@@ -84,7 +92,7 @@ const bj_tune_entry* find_tune(const bj_ctx* c, int kind, int lg, int sr, int se
     for (const bj_tune_entry& e : c->tune) {
         if (e.kind != kind) continue;
         if (kind == 0 && e.lg == lg) return &e;
-        if (kind == 1 && e.sr == sr && e.sel == sel) return &e;
+        if (kind == 1 && e.lg == lg && e.sr == sr && e.sel == sel) return &e;
     }
     return nullptr;
 }
@@ -114,10 +122,12 @@ int resolve(const bj_ctx* c, const bj_join_spec& s, JoinCfg& o)
 
     double bpk = s.bf_bits_per_key;
     if (bpk < 0.0) {
-        const bj_tune_entry* t1 = find_tune(c, 1, 0, sr_class(nr, ns), sel_class(s.match_hint));
+        // the tuner's filter classes: filters that stay in the L2 (lg 0) and
+        // filters probed after a first pass of S (lg 1), told apart by the size
+        // of a 10 bits/key filter
+        const int fcls = filter_class(nr);
+        const bj_tune_entry* t1 = find_tune(c, 1, fcls, sr_class(nr, ns), sel_class(s.match_hint));
         bpk = t1 ? t1->bf_bits_per_key : (sel_class(s.match_hint) <= 1 ? 10.0 : 0.0);
-        // the tuner times its filters on joins whose filter fits the L2
-        if ((double)nr * bpk / 8.0 > (double)B_FILTER_L2_BYTES) bpk = 0.0;
     }
     if (nr == 0 || ns == 0) bpk = 0.0;
     o.bpk = bpk;
@@ -138,6 +148,8 @@ int resolve(const bj_ctx* c, const bj_join_spec& s, JoinCfg& o)
         o.geom.seg_bits = seg_bits;
         o.geom.wps = (uint32_t)wps;
         o.geom.k = o.k;
+        o.bf2 = bytes > B_FILTER_L2_BYTES;
+        o.fb = o.bf2 ? std::max(1, std::min(o.bits, ceil_log2((bytes + B_SLICE_BYTES - 1) / B_SLICE_BYTES))) : 0;
     } else {
         o.k = 0;
     }
@@ -173,7 +185,7 @@ struct Layout {
     uint32_t nt_r = 0, nt_s = 0, nseg = 0, ncol_r = 0, ncol_s = 0, nsb = 0, nsb_a = 0, max_wps = 1;
     uint32_t nt_r2 = 0, nt_s2 = 0;
     int max_bits = 1, max_b1 = 1, max_b2 = 0;
-    bool any_bf = false, any_two = false;
+    bool any_bf = false, any_two = false, any_bf2 = false;
     uint64_t tot_r = 0, tot_s = 0, tot_sp = 0, part_n = 0, f_blocks = 0, out_n = 0, item_cap = 16;
 };
 
@@ -210,6 +222,8 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
         J.bits = g.bits;
         J.b1 = g.b1;
         J.bf = g.bpk > 0.0 ? 1 : 0;
+        J.bf2 = J.bf && g.bf2 ? 1 : 0;
+        J.fb = J.bf2 ? g.fb : 0;
         J.geom = g.geom;
         J.tile = (uint32_t)g.tile;
         const uint64_t nbins = 1ull << g.bits;
@@ -231,6 +245,10 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
             L.max_b2 = std::max(L.max_b2, g.bits - g.b1);
         }
         L.max_b1 = std::max(L.max_b1, g.b1);
+        if (J.bf2) {
+            L.any_bf2 = true;
+            L.max_b1 = std::max(L.max_b1, J.fb);
+        }
         PX(PX_RTASK, i) = (uint32_t)nt_r;
         nt_r += cr;
         PX(PX_STASK, i) = (uint32_t)nt_s;
@@ -321,8 +339,8 @@ int batch_workspace(bj_batch* b, const Layout& L, int n, uint64_t* total_bytes)
         (rc = need(&bj_batch::r_val, L.tot_r * 4)) || (rc = need(&bj_batch::s_key, L.tot_s * 4)) ||
         (rc = need(&bj_batch::s_val, L.tot_s * 4)) || (rc = need(&bj_batch::sp_key, L.tot_sp * 4)) ||
         (rc = need(&bj_batch::sp_val, L.tot_sp * 4)) ||
-        (rc = need(&bj_batch::tmp_key, L.any_two ? std::max(L.tot_r, L.tot_s) * 4 : 0)) ||
-        (rc = need(&bj_batch::tmp_val, L.any_two ? std::max(L.tot_r, L.tot_s) * 4 : 0)) ||
+        (rc = need(&bj_batch::tmp_key, L.any_two || L.any_bf2 ? std::max(L.tot_r, L.tot_s) * 4 : 0)) ||
+        (rc = need(&bj_batch::tmp_val, L.any_two || L.any_bf2 ? std::max(L.tot_r, L.tot_s) * 4 : 0)) ||
         (rc = need(&bj_batch::part, L.part_n * 4)) ||
         (rc = need(&bj_batch::cur_rel, L.part_n * 4)) || (rc = need(&bj_batch::tot_r, nsb * 4)) ||
         (rc = need(&bj_batch::tot_s, nsb * 4)) || (rc = need(&bj_batch::start_r, nsb * 8)) ||
@@ -378,8 +396,21 @@ int enqueue(bj_batch* b, hipStream_t st)
     if (L.any_bf) {
         BJ_HIP(launch_batch_bloom_build(B, L.nseg, L.max_wps, P<uint32_t>(b->r_key), P<uint64_t>(b->start_r),
                                         P<uint32_t>(b->tot_r), P<uint64_t>(b->filter), st));
-        BJ_HIP(launch_batch_prefilter(B, L.nt_s, P<uint64_t>(b->filter), P<uint32_t>(b->sp_key),
-                                      P<uint32_t>(b->sp_val), jc, err, st));
+        if (L.any_bf2) {
+            // the joins with large filters: S by its top fb bits into the scratch
+            // arena first (the S cursors are recomputed for S' below)
+            BJ_HIP(launch_batch_hist(B, 2, L.nt_s, nullptr, jc, P<uint32_t>(b->part), st));
+            BJ_HIP(launch_batch_colscan(B, 2, L.ncol_s, P<uint32_t>(b->part), P<uint32_t>(b->cur_rel),
+                                        P<uint32_t>(b->tot_s), st));
+            BJ_HIP(launch_batch_binscan(B, 2, P<uint32_t>(b->tot_s), P<uint64_t>(b->start_s), st));
+            BJ_HIP(launch_batch_scatter(B, 2, L.nt_s, nullptr, nullptr, jc, P<uint32_t>(b->cur_rel),
+                                        P<uint64_t>(b->start_s), P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
+                                        b->tmp_key.bytes / 4, P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
+                                        b->tmp_key.bytes / 4, err, false, st));
+        }
+        BJ_HIP(launch_batch_prefilter(B, L.nt_s, P<uint64_t>(b->filter), P<uint32_t>(b->tmp_key),
+                                      P<uint32_t>(b->tmp_val), P<uint32_t>(b->sp_key), P<uint32_t>(b->sp_val), jc,
+                                      err, st));
     }
     // S (or S'): one-level partition into the S arena
     BJ_HIP(launch_batch_hist(B, 1, L.nt_s, P<uint32_t>(b->sp_key), jc, P<uint32_t>(b->part), st));
@@ -419,6 +450,8 @@ void fill_cfg(const JoinCfg& g, bj_join_out& o)
     o.radix_bits = g.bits;
     o.tile = g.tile;
     o.bf_k = g.bpk > 0.0 ? g.k : 0;
+    o.bf_stage = g.bpk > 0.0 ? (g.bf2 ? 2 : 1) : 0;
+    o.bf_pass_bits = g.bpk > 0.0 && g.bf2 ? g.fb : 0;
     o.bf_bits_per_key = g.bpk;
     o.capacity = g.cap;
 }
@@ -693,8 +726,8 @@ extern "C" int bj_autotune(bj_ctx* c, int n, const bj_join_spec* specs, int* ent
             if (!find_tune(c, 0, lg, 0, 0)) lgs[lg] = true;
         }
         if (s.bf_bits_per_key < 0.0) {
-            const int sr = sr_class(nr, ns), sel = sel_class(s.match_hint);
-            if (!find_tune(c, 1, 0, sr, sel)) bfs[{sr, sel}] = true;
+            const int sr = sr_class(nr, ns), sel = sel_class(s.match_hint), fcls = filter_class(nr);
+            if (!find_tune(c, 1, fcls, sr, sel)) bfs[{fcls * 16 + sr + 3, sel}] = true;
         }
     }
     TuneData d;
@@ -727,19 +760,22 @@ extern "C" int bj_autotune(bj_ctx* c, int n, const bj_join_spec* specs, int* ent
         c->tune.push_back(best);
         tuned++;
     }
-    // filter bits per key per (|S|/|R|, selectivity) class, at |R| = 2^20
+    // filter bits per key per (filter class, |S|/|R|, selectivity) class: at
+    // |R| = 2^20 for filters that stay in the L2, at 2^23 (two joins) for those
+    // probed after a first pass of S
     for (const auto& kv : bfs) {
         if (rc) break;
-        const int sr = kv.first.first, sel = kv.first.second;
-        const uint64_t nr = 1ull << 20;
+        const int fcls = kv.first.first / 16, sr = kv.first.first % 16 - 3, sel = kv.first.second;
+        const uint64_t nr = fcls ? (1ull << 23) : (1ull << 20);
         const uint64_t ns = sr >= 0 ? (nr << sr) : (nr >> -sr);
-        if ((rc = gen_tune_data(c, d, 8, nr, ns, SEL_REP[sel]))) break;
+        if ((rc = gen_tune_data(c, d, fcls ? 2 : 8, nr, ns, SEL_REP[sel]))) break;
         const int lg = lg_class(std::min(nr, ns));
         const bj_tune_entry* t0 = find_tune(c, 0, lg, 0, 0);
         const int tile = t0 ? t0->tile : 8192;
         const int bits = t0 ? t0->radix_bits : rule_bits(std::min(nr, ns), tile);
         bj_tune_entry best{};
         best.kind = 1;
+        best.lg = fcls;
         best.sr = sr;
         best.sel = sel;
         best.ms = 1e30;

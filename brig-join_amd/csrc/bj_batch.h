@@ -29,6 +29,8 @@ struct BJoin {
     int bits;                 // radix bits (1..15)
     int b1;                   // bits of the first scatter pass (== bits: one pass)
     int bf;                   // Bloom filter on
+    int bf2;                  // ... and probed after a first pass of S by the top fb bits,
+    int fb;                   //     so that each bucket's filter slice stays in the L2
     BloomGeom geom;           // filter geometry (segments = top seg_bits radix bits)
     uint32_t sb_base;         // first global sub-bucket
     uint32_t r_ch, s_ch;      // chunk length of R / S histogram and scatter tasks
@@ -58,7 +60,9 @@ struct BatchDev {
 hipError_t batch_kernels_init(int max_lds_bytes);
 
 // side 0 = R, side 1 = S.  For S of a Bloom join the source is the S' staging
-// arena and its length the device counter jc[b][JC_SPRIME].
+// arena and its length the device counter jc[b][JC_SPRIME].  side 2 = the first
+// pass of S of the joins that filter after it (bf2): their S by its top fb bits
+// into the scratch arena (other joins: nothing), before the filter.
 hipError_t launch_batch_hist(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* sp_key,
                              const uint64_t* jc, uint32_t* part, hipStream_t st);
 hipError_t launch_batch_colscan(const BatchDev& B, int side, uint32_t ncolblk, const uint32_t* part,
@@ -81,9 +85,11 @@ hipError_t launch_batch_scatter2(const BatchDev& B, int side, uint32_t ntasks, c
 hipError_t launch_batch_bloom_build(const BatchDev& B, uint32_t nseg, uint32_t max_wps, const uint32_t* rkey,
                                     const uint64_t* r_start, const uint32_t* r_cnt, uint64_t* filter,
                                     hipStream_t st);
+// S -> S' of every Bloom join, chunk by chunk: from its S (bf) or, bucket by
+// bucket, from its first pass in the scratch arena tkey / tval (bf2).
 hipError_t launch_batch_prefilter(const BatchDev& B, uint32_t ntasks, const uint64_t* filter,
-                                  uint32_t* sp_key, uint32_t* sp_val, uint64_t* jc, uint64_t* err,
-                                  hipStream_t st);
+                                  const uint32_t* tkey, const uint32_t* tval, uint32_t* sp_key,
+                                  uint32_t* sp_val, uint64_t* jc, uint64_t* err, hipStream_t st);
 hipError_t launch_batch_plan(const BatchDev& B, uint32_t nsb, const uint64_t* r_start, const uint32_t* r_cnt,
                              const uint64_t* s_start, const uint32_t* s_cnt, uint32_t slice,
                              uint32_t max_slices, uint32_t* counts, uint64_t* item_start, uint64_t* partials,

@@ -36,6 +36,10 @@ __device__ __forceinline__ const uint32_t* px_of(const BatchDev& B, int which)
     return B.px + (size_t)which * (B.n + 1);
 }
 
+// Radix bits and first-pass bits of one side (side 2: the filter pass of S).
+__device__ __forceinline__ int side_bits(const BJoin& J, int side) { return side == 2 ? J.fb : J.bits; }
+__device__ __forceinline__ int side_b1(const BJoin& J, int side) { return side == 2 ? J.fb : J.b1; }
+
 // Source and current length of one side of join b.
 __device__ __forceinline__ void side_src(const BJoin& J, int b, int side, const uint32_t* sp_key,
                                          const uint32_t* sp_val, const uint64_t* jc, const uint32_t*& key,
@@ -45,6 +49,10 @@ __device__ __forceinline__ void side_src(const BJoin& J, int b, int side, const 
         key = J.rk;
         val = J.rv;
         n = J.nr;
+    } else if (side == 2) {
+        key = J.sk;
+        val = J.sv;
+        n = J.bf2 ? J.ns : 0u;
     } else if (J.bf) {
         key = sp_key + J.sp_base;
         val = sp_val ? sp_val + J.sp_base : nullptr;
@@ -74,7 +82,8 @@ __global__ __launch_bounds__(BLOCK) void kb_hist(BatchDev B, int side, const uin
     const int b = find_join(px, B.n, t);
     const BJoin J = B.J[b];
     const uint32_t c = t - px[b];
-    const int nbins = 1 << J.bits;
+    const int bits = side_bits(J, side);
+    const int nbins = 1 << bits;
     for (int i = threadIdx.x; i < nbins; i += BLOCK) h[i] = 0u;
     __syncthreads();
     const uint32_t *key, *val;
@@ -82,7 +91,7 @@ __global__ __launch_bounds__(BLOCK) void kb_hist(BatchDev B, int side, const uin
     side_src(J, b, side, sp_key, nullptr, jc, key, val, n);
     const uint64_t ch = side ? J.s_ch : J.r_ch;
     const uint64_t lo = min((uint64_t)c * ch, n), hi = min(lo + ch, n);
-    hist_range<BLOCK>(key, lo, hi, 32 - J.bits, h);
+    hist_range<BLOCK>(key, lo, hi, 32 - bits, h);
     __syncthreads();
     uint32_t* dst = part + (side ? J.part_s : J.part_r) + (size_t)c * nbins;
     for (int i = threadIdx.x; i < nbins; i += BLOCK) dst[i] = h[i];
@@ -102,7 +111,7 @@ __global__ __launch_bounds__(1024) void kb_colscan(BatchDev B, int side, const u
     const int b = find_join(px, B.n, t);
     const BJoin J = B.J[b];
     const int cb = (int)(t - px[b]);
-    const int nbins = 1 << J.bits;
+    const int nbins = 1 << side_bits(J, side);
     const int G = (int)side_chunks(J, side);
     const uint64_t px0 = side ? J.part_s : J.part_r;
     const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(1024) void kb_binscan(BatchDev B, int side, const u
 {
     __shared__ uint64_t wtmp[1024 / WAVE + 1];
     const BJoin J = B.J[blockIdx.x];
-    const int nbins = 1 << J.bits;
+    const int nbins = 1 << side_bits(J, side);
     const int per = (nbins + 1023) / 1024;
     const int lo = min((int)threadIdx.x * per, nbins), hi = min(lo + per, nbins);
     const uint32_t* tt = tot + J.sb_base;
@@ -187,11 +196,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ITEMS == 
     const int b = find_join(px, B.n, t);
     const BJoin J = B.J[b];
     const uint32_t c = t - px[b];
-    const int b2 = J.bits - J.b1;
-    const int nbk = 1 << J.b1;
+    const int bits = side_bits(J, side), b1 = side_b1(J, side);
+    const int b2 = bits - b1;
+    const int nbk = 1 << b1;
     const int cs_log2 = kb_carry_log2(nbk, ITEMS == 8);
     const ScatterLds L = scatter_lds<BLOCK, ITEMS>(lds64, nbk, cs_log2);
-    const uint32_t* cr = cur_rel + (side ? J.part_s : J.part_r) + ((size_t)c << J.bits);
+    const uint32_t* cr = cur_rel + (side ? J.part_s : J.part_r) + ((size_t)c << bits);
     const uint64_t* st = start + J.sb_base;
     // cursor of bucket j: st[j << b2] + its 2^b2 entries of cr, summed by one wave
     // (coalesced across lanes) or, for buckets narrower than a wave, by aligned
@@ -224,7 +234,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ITEMS == 
     const uint64_t ch = side ? J.s_ch : J.r_ch;
     const uint64_t lo = min((uint64_t)c * ch, n), hi = min(lo + ch, n);
     const bool two = b2 > 0;
-    const bool bad = scatter_range<BLOCK, ITEMS>(L, key + lo, val + lo, (uint32_t)(hi - lo), nbk, 32 - J.b1,
+    const bool bad = scatter_range<BLOCK, ITEMS>(L, key + lo, val + lo, (uint32_t)(hi - lo), nbk, 32 - b1,
                                                  (uint32_t)nbk - 1u, cs_log2, two ? tkey : okey, two ? tval : oval,
                                                  two ? tmp_cap : out_cap);
     if (bad) atomicOr((unsigned long long*)err, 16ull);
@@ -313,8 +323,12 @@ __global__ __launch_bounds__(BLOCK) void kb_bloom_build(BatchDev B, const uint32
 // ============================================================== Bloom filter S -> S'
 // The S chunk tasks of every Bloom join filter their chunk tile by tile into
 // the join's S' staging region (per-join survivor counter).
+// A bf2 join's chunks read its first pass (scratch arena, grouped by the top fb
+// bits), so the keys of a chunk probe one or two L2-sized slices of the filter.
 template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void kb_prefilter(BatchDev B, const uint64_t* __restrict__ filter,
+                                                      const uint32_t* __restrict__ tkey,
+                                                      const uint32_t* __restrict__ tval,
                                                       uint32_t* __restrict__ sp_key, uint32_t* __restrict__ sp_val,
                                                       uint64_t* __restrict__ jc, uint64_t* __restrict__ err)
 {
@@ -329,8 +343,10 @@ __global__ __launch_bounds__(BLOCK) void kb_prefilter(BatchDev B, const uint64_t
     const uint64_t c = t - px[b];
     const uint64_t lo = min(c * J.s_ch, (uint64_t)J.ns), hi = min(lo + J.s_ch, (uint64_t)J.ns);
     bool bad = false;
+    const uint32_t* sk = J.bf2 ? tkey + J.s_base : J.sk;
+    const uint32_t* sv = J.bf2 ? tval + J.s_base : J.sv;
     for (uint64_t tile = lo; tile < hi; tile += T)
-        bad |= prefilter_tile<BLOCK, ITEMS>(J.sk, J.sv, tile, hi, filter + J.f_base, J.geom, sp_key + J.sp_base,
+        bad |= prefilter_tile<BLOCK, ITEMS>(sk, sv, tile, hi, filter + J.f_base, J.geom, sp_key + J.sp_base,
                                             sp_val + J.sp_base, J.ns, jc + (size_t)b * JC_N + JC_SPRIME, gcnt,
                                             &s_base);
     if (bad) atomicOr((unsigned long long*)err, 32ull);
@@ -538,11 +554,11 @@ hipError_t launch_batch_bloom_build(const BatchDev& B, uint32_t nseg, uint32_t m
 }
 
 hipError_t launch_batch_prefilter(const BatchDev& B, uint32_t ntasks, const uint64_t* filter,
-                                  uint32_t* sp_key, uint32_t* sp_val, uint64_t* jc, uint64_t* err,
-                                  hipStream_t st)
+                                  const uint32_t* tkey, const uint32_t* tval, uint32_t* sp_key,
+                                  uint32_t* sp_val, uint64_t* jc, uint64_t* err, hipStream_t st)
 {
     if (ntasks == 0) return hipSuccess;
-    kb_prefilter<512, 8><<<ntasks, 512, 0, st>>>(B, filter, sp_key, sp_val, jc, err);
+    kb_prefilter<512, 8><<<ntasks, 512, 0, st>>>(B, filter, tkey, tval, sp_key, sp_val, jc, err);
     return hipGetLastError();
 }
 

@@ -385,6 +385,10 @@ typedef struct bj_join_out {
     uint64_t s_survivors;            /* |S'| after the filter (= |S| without)    */
     int32_t  truncated;
     int32_t  radix_bits, tile, bf_k; /* the configuration used                   */
+    int32_t  bf_stage;               /* 0: no filter; 1: S filtered before its
+                                        partition; 2: after a first pass of S by
+                                        its top bf_pass_bits bits (large filters) */
+    int32_t  bf_pass_bits;
     double   bf_bits_per_key;        /* 0 = filter off                           */
     /* rows (SoA, device, unordered), owned by the batch until bj_batch_destroy */
     uint32_t *key, *r_val, *s_val;
@@ -393,7 +397,10 @@ typedef struct bj_join_out {
 /* One autotuner table entry.  kind 0: radix bits and tile for joins whose
  * smaller side is about 2^lg tuples; kind 1: filter bits per key for joins with
  * |S|/|R| about 2^sr and selectivity class sel (0: <= 3%, 1: <= 15%, 2: <= 35%,
- * 3: more). */
+ * 3: more), in filter class lg (0: a 10 bits/key filter of R stays in an XCD's
+ * L2, 3.5 MB, and S is filtered before its partition; 1: a larger filter, probed
+ * after a first pass of S by the top bits, so each bucket probes an L2-sized
+ * slice). */
 typedef struct bj_tune_entry {
     int32_t kind;
     int32_t lg, sr, sel;

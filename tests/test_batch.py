@@ -49,16 +49,23 @@ def test_plan_selectivity_hint_rule():
     assert [c.bf_bits_per_key > 0 for c in cfg] == [True, True, False, False]
 
 
-def test_plan_filter_only_within_l2():
-    """A filter the planner chooses itself is kept only while it fits an XCD's L2
-    (3.5 MB): the batch probes it at random before partitioning S.  An explicit
-    filter is used as given."""
+def test_plan_filter_stage_by_size():
+    """A filter that fits an XCD's L2 (3.5 MB) is probed at random before S is
+    partitioned (stage 1); a larger one after a first pass of S by enough top
+    bits that each bucket probes a slice of at most 128 KB (stage 2), never more
+    bits than the join's own radix bits."""
     cfg, _ = bj.batch_plan([spec(2_800_000, 4 << 20, match_hint=0.01),   # 3.5 MB at 10 bits/key
                             spec(3_000_000, 4 << 20, match_hint=0.01),   # 3.75 MB
-                            spec(16 << 20, 16 << 20, match_hint=0.01),
-                            spec(16 << 20, 16 << 20, match_hint=0.01, bf_bits_per_key=10.0)])
-    assert [c.bf_bits_per_key for c in cfg] == [10.0, 0.0, 0.0, 10.0]
-    assert [c.bf_k > 0 for c in cfg] == [True, False, False, True]
+                            spec(16 << 20, 16 << 20, match_hint=0.01),   # 20 MB
+                            spec(16 << 20, 16 << 20, match_hint=0.01, bf_bits_per_key=10.0),
+                            spec(16 << 20, 16 << 20, match_hint=0.01, bf_bits_per_key=10.0, radix_bits=7),
+                            spec(16 << 20, 16 << 20, match_hint=0.5)])
+    assert [c.bf_bits_per_key for c in cfg] == [10.0, 10.0, 10.0, 10.0, 10.0, 0.0]
+    assert [c.bf_stage for c in cfg] == [1, 2, 2, 2, 2, 0]
+    assert cfg[1].bf_pass_bits == math.ceil(math.log2(3_000_000 * 10 / 8 / (128 << 10)))
+    assert cfg[2].bf_pass_bits == math.ceil(math.log2((16 << 20) * 10 / 8 / (128 << 10)))
+    assert cfg[4].bf_pass_bits == 7 and cfg[4].radix_bits == 7
+    assert [c.bf_pass_bits for c in (cfg[0], cfg[5])] == [0, 0]
 
 
 @pytest.mark.parametrize("kw,code", [
@@ -154,6 +161,41 @@ def test_batch_matches_oracle(gpu_ctx, use_graph):
             assert np.array_equal(_rows(b, o), O.sort_rows(k, a, c)), i
             if o.bf_k:
                 assert o.s_survivors <= len(j.h[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_batch_large_filters_after_first_pass(gpu_ctx, use_graph):
+    """Filters larger than an XCD's L2 are probed after a first pass of S by the
+    top bits (stage 2), next to stage-1 and unfiltered joins of the same batch;
+    every join's rows equal the oracle's, repeated S keys included."""
+    joins = []
+    for i, (nr, ns, kw) in enumerate([
+            (3_000_000, 6_000_000, dict(bf_bits_per_key=10.0)),                 # 3.75 MB: stage 2
+            (4_500_000, 5_000_000, dict(bf_bits_per_key=12.0, num_hashes=4)),   # 6.75 MB
+            (20_000, 50_000, dict(bf_bits_per_key=10.0)),                       # stage 1
+            (3_200_000, 3_300_000, dict(bf_bits_per_key=10.0, tile=4096)),
+            (100_000, 400_000, dict(bf_bits_per_key=0.0))]):                    # no filter
+        rk, rv = O.gen_r(300 + i, nr)
+        m = O.match_count(ns, 0.05)
+        sk, sv = O.gen_s(300 + i, 301 + i, nr, ns, m)
+        if i == 1:
+            # a hot key: 40000 matching S positions carry R's key of row 77
+            pos = np.nonzero(sk < np.uint32(1 << 31))[0][:40000]
+            sk[pos] = rk[77]
+        joins.append(HostJoin(rk, rv, sk, sv, **kw))
+    print(f"[batch stage 2] joins={len(joins)} graph={use_graph}", flush=True)
+    with bj.Batch(gpu_ctx, [j.spec() for j in joins], use_graph=bool(use_graph)) as b:
+        for rep in range(2):
+            outs, _ = b.run()
+            assert [o.bf_stage for o in outs] == [2, 2, 1, 2, 0]
+            assert all(o.bf_pass_bits >= 1 for o in (outs[0], outs[1], outs[3]))
+            for i, (j, o) in enumerate(zip(joins, outs)):
+                n, k, a, c = O.cpu_join(*j.h)
+                assert o.match_count == n and not o.truncated, (rep, i)
+                assert np.array_equal(_rows(b, o), O.sort_rows(k, a, c)), (rep, i)
+                if o.bf_k:
+                    assert n <= o.s_survivors <= len(j.h[2]), (rep, i)
 
 
 @pytest.mark.gpu
