@@ -183,7 +183,7 @@ struct Layout {
     std::vector<JoinCfg> cfg;
     std::vector<uint32_t> px; // PX_N x (n + 1)
     uint32_t nt_r = 0, nt_s = 0, nseg = 0, ncol_r = 0, ncol_s = 0, nsb = 0, nsb_a = 0, max_wps = 1;
-    uint32_t nt_r2 = 0, nt_s2 = 0;
+    uint32_t nt_r2 = 0, nt_s2 = 0, nf_tiles = 0;
     int max_bits = 1, max_b1 = 1, max_b2 = 0;
     bool any_bf = false, any_two = false, any_bf2 = false;
     uint64_t tot_r = 0, tot_s = 0, tot_sp = 0, part_n = 0, f_blocks = 0, out_n = 0, item_cap = 16;
@@ -208,6 +208,7 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
     L.px.assign((size_t)PX_N * (n + 1), 0u);
     auto PX = [&](int which, int i) -> uint32_t& { return L.px[(size_t)which * (n + 1) + i]; };
     uint64_t nt_r = 0, nt_s = 0, nseg = 0, ncol_r = 0, ncol_s = 0, nsb = 0, nsb_a = 0, nt_r2 = 0, nt_s2 = 0;
+    uint64_t nft = 0;
     for (int i = 0; i < n; i++) {
         const bj_join_spec& s = specs[L.order[i]];
         const JoinCfg& g = cfg[L.order[i]];
@@ -266,7 +267,9 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
         J.s_base = L.tot_s;
         L.tot_s += J.ns;
         PX(PX_SEG, i) = (uint32_t)nseg;
+        PX(PX_FTILE, i) = (uint32_t)nft;
         if (J.bf) {
+            nft += (J.ns + (uint64_t)PF_TILE - 1) / PF_TILE;
             J.sp_base = L.tot_sp;
             L.tot_sp += J.ns;
             J.f_base = L.f_blocks;
@@ -289,7 +292,8 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
     PX(PX_SB, n) = (uint32_t)nsb;
     PX(PX_R2, n) = (uint32_t)nt_r2;
     PX(PX_S2, n) = (uint32_t)nt_s2;
-    if (std::max({nt_r, nt_s, nseg, ncol_r, ncol_s, nsb, nt_r2, nt_s2}) > 0x7FFFFFFFull)
+    PX(PX_FTILE, n) = (uint32_t)nft;
+    if (std::max({nt_r, nt_s, nseg, ncol_r, ncol_s, nsb, nt_r2, nt_s2, nft}) > 0x7FFFFFFFull)
         return fail(BJ_ERR_SIZE, "batch too large for one launch per phase");
     L.nt_r = (uint32_t)nt_r;
     L.nt_s = (uint32_t)nt_s;
@@ -300,6 +304,7 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
     L.nsb_a = (uint32_t)nsb_a;
     L.nt_r2 = (uint32_t)nt_r2;
     L.nt_s2 = (uint32_t)nt_s2;
+    L.nf_tiles = (uint32_t)nft;
     return BJ_OK;
 }
 
@@ -320,8 +325,9 @@ struct bj_batch {
 
 namespace {
 
-// counters: jc[n][JC_N], work counters of the two probe launches, error flags
-size_t ctr_words(int n) { return (size_t)n * JC_N + 3; }
+// counters: jc[n][JC_N], work counters of the two probe launches, error flags,
+// the filter launch's per-XCD tile counters
+size_t ctr_words(int n) { return (size_t)n * JC_N + 3 + NXCD; }
 
 // Sums (b == nullptr) or allocates the workspace of a batch.
 int batch_workspace(bj_batch* b, const Layout& L, int n, uint64_t* total_bytes)
@@ -408,9 +414,9 @@ int enqueue(bj_batch* b, hipStream_t st)
                                         b->tmp_key.bytes / 4, P<uint32_t>(b->tmp_key), P<uint32_t>(b->tmp_val),
                                         b->tmp_key.bytes / 4, err, false, st));
         }
-        BJ_HIP(launch_batch_prefilter(B, L.nt_s, P<uint64_t>(b->filter), P<uint32_t>(b->tmp_key),
+        BJ_HIP(launch_batch_prefilter(B, L.nf_tiles, P<uint64_t>(b->filter), P<uint32_t>(b->tmp_key),
                                       P<uint32_t>(b->tmp_val), P<uint32_t>(b->sp_key), P<uint32_t>(b->sp_val), jc,
-                                      err, st));
+                                      err + 1, err, num_cu, st));
     }
     // S (or S'): one-level partition into the S arena
     BJ_HIP(launch_batch_hist(B, 1, L.nt_s, P<uint32_t>(b->sp_key), jc, P<uint32_t>(b->part), st));

@@ -47,7 +47,11 @@ struct BJoin {
 enum { JC_SPRIME = 0, JC_OUT = 1, JC_N = 2 };
 
 // Prefix arrays (n + 1 entries each, u32) that map a task index to its join.
-enum { PX_RTASK = 0, PX_STASK, PX_SEG, PX_RCOL, PX_SCOL, PX_SB, PX_R2, PX_S2, PX_N };
+enum { PX_RTASK = 0, PX_STASK, PX_SEG, PX_RCOL, PX_SCOL, PX_SB, PX_R2, PX_S2, PX_FTILE, PX_N };
+
+// Keys per tile of the S -> S' filter launch (PX_FTILE counts these tiles).
+constexpr int PF_BLOCK = 512, PF_ITEMS = 8, PF_TILE = PF_BLOCK * PF_ITEMS;
+constexpr int NXCD = 8;  // XCDs of the chip: workgroup i of a launch runs on XCD i % 8
 
 struct BatchDev {
     const BJoin* J;           // [n]
@@ -87,9 +91,13 @@ hipError_t launch_batch_bloom_build(const BatchDev& B, uint32_t nseg, uint32_t m
                                     hipStream_t st);
 // S -> S' of every Bloom join, chunk by chunk: from its S (bf) or, bucket by
 // bucket, from its first pass in the scratch arena tkey / tval (bf2).
-hipError_t launch_batch_prefilter(const BatchDev& B, uint32_t ntasks, const uint64_t* filter,
+// S -> S' of every Bloom join: ntiles filter tiles (PX_FTILE), persistent,
+// XCD x working through the x-th eighth of them on its own counter xctr[x]
+// (NXCD counters, zeroed).
+hipError_t launch_batch_prefilter(const BatchDev& B, uint32_t ntiles, const uint64_t* filter,
                                   const uint32_t* tkey, const uint32_t* tval, uint32_t* sp_key,
-                                  uint32_t* sp_val, uint64_t* jc, uint64_t* err, hipStream_t st);
+                                  uint32_t* sp_val, uint64_t* jc, uint64_t* xctr, uint64_t* err, int num_cu,
+                                  hipStream_t st);
 hipError_t launch_batch_plan(const BatchDev& B, uint32_t nsb, const uint64_t* r_start, const uint32_t* r_cnt,
                              const uint64_t* s_start, const uint32_t* s_cnt, uint32_t slice,
                              uint32_t max_slices, uint32_t* counts, uint64_t* item_start, uint64_t* partials,

@@ -321,34 +321,53 @@ __global__ __launch_bounds__(BLOCK) void kb_bloom_build(BatchDev B, const uint32
 }
 
 // ============================================================== Bloom filter S -> S'
-// The S chunk tasks of every Bloom join filter their chunk tile by tile into
-// the join's S' staging region (per-join survivor counter).
-// A bf2 join's chunks read its first pass (scratch arena, grouped by the top fb
-// bits), so the keys of a chunk probe one or two L2-sized slices of the filter.
+// Persistent over the filter tiles of all Bloom joins (PF_TILE keys each,
+// numbered join by join).  Workgroups are dealt to the XCDs round-robin, and
+// XCD x takes the x-th eighth of the tile numbers in order from its own
+// counter xctr[x]: the workgroups of one XCD work on neighbouring tiles of one
+// or two joins at a time, so those filters stay in that XCD's L2 (dealt out by
+// workgroup index instead, every XCD's L2 would hold the filters of every join
+// in flight).  Each tile reserves its survivors' places in its join's S'
+// staging region.  A bf2 join's tiles read its first pass (scratch arena,
+// grouped by the top fb bits), so the keys of a tile probe one or two L2-sized
+// slices of the filter.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void kb_prefilter(BatchDev B, const uint64_t* __restrict__ filter,
+__global__ __launch_bounds__(BLOCK) void kb_prefilter(BatchDev B, uint32_t ntiles,
+                                                      const uint64_t* __restrict__ filter,
                                                       const uint32_t* __restrict__ tkey,
                                                       const uint32_t* __restrict__ tval,
                                                       uint32_t* __restrict__ sp_key, uint32_t* __restrict__ sp_val,
-                                                      uint64_t* __restrict__ jc, uint64_t* __restrict__ err)
+                                                      uint64_t* __restrict__ jc, uint64_t* __restrict__ xctr,
+                                                      uint64_t* __restrict__ err)
 {
     constexpr int T = BLOCK * ITEMS;
     __shared__ uint32_t gcnt[ITEMS * (BLOCK / WAVE)];
     __shared__ uint64_t s_base;
-    const uint32_t* px = px_of(B, PX_STASK);
-    const uint32_t t = blockIdx.x;
-    const int b = find_join(px, B.n, t);
-    const BJoin J = B.J[b];
-    if (!J.bf) return;  // uniform per workgroup
-    const uint64_t c = t - px[b];
-    const uint64_t lo = min(c * J.s_ch, (uint64_t)J.ns), hi = min(lo + J.s_ch, (uint64_t)J.ns);
+    __shared__ uint32_t s_tile, s_next;
+    const uint32_t* px = px_of(B, PX_FTILE);
+    const uint32_t xcd = blockIdx.x % NXCD;
+    const uint32_t lo = (uint32_t)(((uint64_t)ntiles * xcd) / NXCD);
+    const uint32_t cnt = (uint32_t)(((uint64_t)ntiles * (xcd + 1)) / NXCD) - lo;
+    unsigned long long* ctr = (unsigned long long*)xctr + xcd;
+    if (threadIdx.x == 0) s_tile = (uint32_t)min(atomicAdd(ctr, 1ull), (unsigned long long)cnt);
+    __syncthreads();
+    uint32_t t = s_tile;
     bool bad = false;
-    const uint32_t* sk = J.bf2 ? tkey + J.s_base : J.sk;
-    const uint32_t* sv = J.bf2 ? tval + J.s_base : J.sv;
-    for (uint64_t tile = lo; tile < hi; tile += T)
-        bad |= prefilter_tile<BLOCK, ITEMS>(sk, sv, tile, hi, filter + J.f_base, J.geom, sp_key + J.sp_base,
+    while (t < cnt) {
+        // the next tile's number, read after the tile's first barrier
+        if (threadIdx.x == 0) s_next = (uint32_t)min(atomicAdd(ctr, 1ull), (unsigned long long)cnt);
+        const uint32_t g = lo + t;
+        const int b = find_join(px, B.n, g);
+        const BJoin J = B.J[b];
+        const uint64_t tile = (uint64_t)(g - px[b]) * T;
+        const uint32_t* sk = J.bf2 ? tkey + J.s_base : J.sk;
+        const uint32_t* sv = J.bf2 ? tval + J.s_base : J.sv;
+        bad |= prefilter_tile<BLOCK, ITEMS>(sk, sv, tile, J.ns, filter + J.f_base, J.geom, sp_key + J.sp_base,
                                             sp_val + J.sp_base, J.ns, jc + (size_t)b * JC_N + JC_SPRIME, gcnt,
                                             &s_base);
+        t = s_next;
+        __syncthreads();  // s_next is rewritten for the next tile
+    }
     if (bad) atomicOr((unsigned long long*)err, 32ull);
 }
 
@@ -553,12 +572,16 @@ hipError_t launch_batch_bloom_build(const BatchDev& B, uint32_t nseg, uint32_t m
     return hipGetLastError();
 }
 
-hipError_t launch_batch_prefilter(const BatchDev& B, uint32_t ntasks, const uint64_t* filter,
+hipError_t launch_batch_prefilter(const BatchDev& B, uint32_t ntiles, const uint64_t* filter,
                                   const uint32_t* tkey, const uint32_t* tval, uint32_t* sp_key,
-                                  uint32_t* sp_val, uint64_t* jc, uint64_t* err, hipStream_t st)
+                                  uint32_t* sp_val, uint64_t* jc, uint64_t* xctr, uint64_t* err, int num_cu,
+                                  hipStream_t st)
 {
-    if (ntasks == 0) return hipSuccess;
-    kb_prefilter<512, 8><<<ntasks, 512, 0, st>>>(B, filter, tkey, tval, sp_key, sp_val, jc, err);
+    if (ntiles == 0) return hipSuccess;
+    // four workgroups per CU, a multiple of NXCD
+    const uint32_t grid = (uint32_t)std::max(1, num_cu / NXCD) * 4u * NXCD;
+    kb_prefilter<PF_BLOCK, PF_ITEMS><<<grid, PF_BLOCK, 0, st>>>(B, ntiles, filter, tkey, tval, sp_key, sp_val, jc,
+                                                                xctr, err);
     return hipGetLastError();
 }
 
