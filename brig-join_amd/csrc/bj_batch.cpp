@@ -239,7 +239,10 @@ int make_layout(const bj_ctx* c, int n, const bj_join_spec* specs, Layout& L)
         PX(PX_S2, i) = (uint32_t)nt_s2;
         if (g.b1 < g.bits) {
             J.w2_r = pass2_groups(J.nr, g.b1, cr);
-            J.w2_s = pass2_groups(J.ns, g.b1, cs);
+            // a Bloom join's second pass sees S', about the hinted match ratio of S
+            // plus false positives (fewer groups than S' needs only costs parallelism)
+            const double sel = (J.bf && s.match_hint >= 0.0) ? std::min(1.0, s.match_hint + 0.05) : 1.0;
+            J.w2_s = pass2_groups((uint64_t)((double)J.ns * sel), g.b1, cs);
             nt_r2 += (uint64_t)J.w2_r << g.b1;
             nt_s2 += (uint64_t)J.w2_s << g.b1;
             L.any_two = true;

@@ -71,6 +71,25 @@ __device__ __forceinline__ uint32_t side_chunks(const BJoin& J, int side)
     return (uint32_t)((n + ch - 1) / ch);
 }
 
+// Tuples [lo, hi) of chunk c of a side of current length n.  The chunks of a
+// Bloom join's S' split its device-side length evenly (S' is known only on the
+// device and is usually a small part of S: chunks of the planned length would
+// leave most tasks empty and the first few with all of it); other sides take
+// chunks of the planned length (multiples of the scatter tile).
+__device__ __forceinline__ void chunk_range(const BJoin& J, int side, uint32_t c, uint64_t n, uint64_t& lo,
+                                            uint64_t& hi)
+{
+    if (side == 1 && J.bf) {
+        const uint64_t G = side_chunks(J, 1);
+        lo = (uint64_t)c * n / G;
+        hi = (uint64_t)(c + 1) * n / G;
+    } else {
+        const uint64_t ch = side ? J.s_ch : J.r_ch;
+        lo = min((uint64_t)c * ch, n);
+        hi = min(lo + ch, n);
+    }
+}
+
 // ============================================================== histogram
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void kb_hist(BatchDev B, int side, const uint32_t* __restrict__ sp_key,
@@ -89,8 +108,8 @@ __global__ __launch_bounds__(BLOCK) void kb_hist(BatchDev B, int side, const uin
     const uint32_t *key, *val;
     uint64_t n;
     side_src(J, b, side, sp_key, nullptr, jc, key, val, n);
-    const uint64_t ch = side ? J.s_ch : J.r_ch;
-    const uint64_t lo = min((uint64_t)c * ch, n), hi = min(lo + ch, n);
+    uint64_t lo, hi;
+    chunk_range(J, side, c, n, lo, hi);
     hist_range<BLOCK>(key, lo, hi, 32 - bits, h);
     __syncthreads();
     uint32_t* dst = part + (side ? J.part_s : J.part_r) + (size_t)c * nbins;
@@ -231,8 +250,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ITEMS == 
     const uint32_t *key, *val;
     uint64_t n;
     side_src(J, b, side, sp_key, sp_val, jc, key, val, n);
-    const uint64_t ch = side ? J.s_ch : J.r_ch;
-    const uint64_t lo = min((uint64_t)c * ch, n), hi = min(lo + ch, n);
+    uint64_t lo, hi;
+    chunk_range(J, side, c, n, lo, hi);
     const bool two = b2 > 0;
     const bool bad = scatter_range<BLOCK, ITEMS>(L, key + lo, val + lo, (uint32_t)(hi - lo), nbk, 32 - b1,
                                                  (uint32_t)nbk - 1u, cs_log2, two ? tkey : okey, two ? tval : oval,
