@@ -47,7 +47,11 @@ struct BJoin {
 enum { JC_SPRIME = 0, JC_OUT = 1, JC_N = 2 };
 
 // Prefix arrays (n + 1 entries each, u32) that map a task index to its join.
-enum { PX_RTASK = 0, PX_STASK, PX_SEG, PX_RCOL, PX_SCOL, PX_SB, PX_R2, PX_S2, PX_FTILE, PX_N };
+// PX_P1 + 2 * side + cls: the first-pass scatter tasks of one side (0 R, 1 S,
+// 2 the filter pass of S) over the joins of one class (0: first passes of fewer
+// than 2^WIDE_B1 buckets, 1: the others), each class in a launch of its own.
+enum { PX_RTASK = 0, PX_STASK, PX_SEG, PX_RCOL, PX_SCOL, PX_SB, PX_R2, PX_S2, PX_FTILE, PX_P1, PX_N = PX_P1 + 6 };
+constexpr int WIDE_B1 = 7;
 
 // Keys per tile of the S -> S' filter launch (PX_FTILE counts these tiles).
 constexpr int PF_BLOCK = 512, PF_ITEMS = 8, PF_TILE = PF_BLOCK * PF_ITEMS;
@@ -56,9 +60,10 @@ constexpr int NXCD = 8;  // XCDs of the chip: workgroup i of a launch runs on XC
 struct BatchDev {
     const BJoin* J;           // [n]
     const uint32_t* px;       // [PX_N][n + 1]
+    const uint32_t* sbj;      // [number of sub-buckets] the join of each sub-bucket
     int n;
     int max_bits;             // largest radix bits of the batch
-    int max_b1, max_b2;       // largest bits of the first / second scatter pass
+    int max_b2;               // largest bits of the second scatter pass
 };
 
 hipError_t batch_kernels_init(int max_lds_bytes);
@@ -75,11 +80,15 @@ hipError_t launch_batch_binscan(const BatchDev& B, int side, const uint32_t* tot
                                 hipStream_t st);
 // First pass: every join's chunks by its first-pass bits, to the final arena
 // (one-pass joins) or to the scratch arena tkey/tval at the final positions.
-hipError_t launch_batch_scatter(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* sp_key,
-                                const uint32_t* sp_val, const uint64_t* jc, const uint32_t* cur_rel,
-                                const uint64_t* start, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
-                                uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err, bool big,
-                                hipStream_t st);
+// One launch per class cls of first passes (ntasks tasks, PX_P1 + 2 * side + cls),
+// with LDS for max_b1 bits.  big: one workgroup per CU with 8192-tuple tiles and
+// whole-line carries (the class of wide first passes), else two per CU with
+// 4096-tuple tiles.
+hipError_t launch_batch_scatter(const BatchDev& B, int side, int cls, uint32_t ntasks, int max_b1,
+                                const uint32_t* sp_key, const uint32_t* sp_val, const uint64_t* jc,
+                                const uint32_t* cur_rel, const uint64_t* start, uint32_t* okey, uint32_t* oval,
+                                uint64_t out_cap, uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err,
+                                bool big, hipStream_t st);
 // Second pass of two-pass joins: scratch arena -> final arena.  big: one
 // workgroup per CU with 8192-tuple tiles (for launches over many tuples).
 hipError_t launch_batch_scatter2(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* cur_rel,
@@ -89,8 +98,6 @@ hipError_t launch_batch_scatter2(const BatchDev& B, int side, uint32_t ntasks, c
 hipError_t launch_batch_bloom_build(const BatchDev& B, uint32_t nseg, uint32_t max_wps, const uint32_t* rkey,
                                     const uint64_t* r_start, const uint32_t* r_cnt, uint64_t* filter,
                                     hipStream_t st);
-// S -> S' of every Bloom join, chunk by chunk: from its S (bf) or, bucket by
-// bucket, from its first pass in the scratch arena tkey / tval (bf2).
 // S -> S' of every Bloom join: ntiles filter tiles (PX_FTILE), persistent,
 // XCD x working through the x-th eighth of them on its own counter xctr[x]
 // (NXCD counters, zeroed).

@@ -191,8 +191,10 @@ __global__ __launch_bounds__(1024) void kb_binscan(BatchDev B, int side, const u
 // start[B << b2] + sum_j cur_rel[c][(B << b2) + j].  One-pass joins (b1 == bits)
 // write the final arena, two-pass joins the scratch arena at the same
 // positions.
-// ITEMS 4: two workgroups per CU, 4096-tuple tiles; ITEMS 8: one per CU (large
-// launches), 8192-tuple tiles and whole-line carries up to 256 buckets.
+// Tasks come from the prefix of the launch's class (cls 0: first passes of fewer
+// than 2^WIDE_B1 buckets, 1: the others).  ITEMS 4: two workgroups per CU,
+// 4096-tuple tiles; ITEMS 8: one per CU, 8192-tuple tiles and whole-line carries
+// up to 256 buckets (the wide class, and large second passes).
 __device__ __forceinline__ int kb_carry_log2(int nbk, bool big)
 {
     if (big && nbk <= 256) return 5;
@@ -200,7 +202,7 @@ __device__ __forceinline__ int kb_carry_log2(int nbk, bool big)
 }
 
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ITEMS == 4 ? 8 : 4, 8))) void kb_scatter(BatchDev B, int side, const uint32_t* __restrict__ sp_key,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ITEMS == 4 ? 8 : 4, 8))) void kb_scatter(BatchDev B, int side, int cls, const uint32_t* __restrict__ sp_key,
                                                     const uint32_t* __restrict__ sp_val,
                                                     const uint64_t* __restrict__ jc,
                                                     const uint32_t* __restrict__ cur_rel,
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ITEMS == 
                                                     uint64_t tmp_cap, uint64_t* __restrict__ err)
 {
     extern __shared__ uint64_t lds64[];
-    const uint32_t* px = px_of(B, side ? PX_STASK : PX_RTASK);
+    const uint32_t* px = px_of(B, PX_P1 + 2 * side + cls);
     const uint32_t t = blockIdx.x;
     const int b = find_join(px, B.n, t);
     const BJoin J = B.J[b];
@@ -397,7 +399,7 @@ __global__ void kb_plan_count(BatchDev B, uint32_t nsb, const uint32_t* __restri
 {
     const uint32_t sb = blockIdx.x * blockDim.x + threadIdx.x;
     if (sb >= nsb) return;
-    const int b = find_join(px_of(B, PX_SB), B.n, sb);
+    const int b = (int)B.sbj[sb];
     uint32_t c, sl;
     n_items[sb] = plan_items(r_cnt[sb], s_cnt[sb], B.J[b].tile, slice, max_slices, c, sl);
 }
@@ -409,7 +411,7 @@ __global__ void kb_plan_fill(BatchDev B, uint32_t nsb, const uint32_t* __restric
 {
     const uint32_t sb = blockIdx.x * blockDim.x + threadIdx.x;
     if (sb >= nsb) return;
-    const int b = find_join(px_of(B, PX_SB), B.n, sb);
+    const int b = (int)B.sbj[sb];
     uint32_t chunks, slices;
     const uint32_t n = plan_items(r_cnt[sb], s_cnt[sb], B.J[b].tile, slice, max_slices, chunks, slices);
     const uint64_t base = item_start[sb];
@@ -431,9 +433,12 @@ __global__ void kb_plan_fill(BatchDev B, uint32_t nsb, const uint32_t* __restric
 // ============================================================== build + probe
 // Persistent over the sub-buckets [sb_lo, sb_hi) of all joins of one tile
 // class: first their vlsplit items, then one item per regular sub-bucket.
-// Rows of join b go to its output region, reserved on its own counter.
+// Rows of join b go to its output region, reserved on its own counter.  Items
+// are claimed KB_PROBE_GRAB at a time (thread 0 waits for the queue's atomic
+// once per claim), and an item's join is one load (B.sbj), beside its sizes.
 // Probe keys per thread per round (as the single join's probe).
 constexpr int KB_PROBE_ITEMS = 10;
+constexpr int KB_PROBE_GRAB = 2;
 
 template <int BLOCK, int CAP, int PITEMS>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_probe(
@@ -449,15 +454,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     const uint64_t i0 = min(item_start[sb_lo], item_cap);
     const uint64_t n_heavy = min(item_start[sb_hi], item_cap) - i0;
     const uint64_t n_work = n_heavy + (uint64_t)(sb_hi - sb_lo);
-    const uint32_t* pxsb = px_of(B, PX_SB);
-
-    if (threadIdx.x == 0) s_item = atomicAdd((unsigned long long*)work_ctr, 1ull);
+    if (threadIdx.x == 0) s_item = atomicAdd((unsigned long long*)work_ctr, (unsigned long long)KB_PROBE_GRAB);
     __syncthreads();
     uint64_t item = s_item;
+    int left = KB_PROBE_GRAB;  // items of the current claim, this one included
     __syncthreads();
 
     while (item < n_work) {
-        if (threadIdx.x == 0) s_next = atomicAdd((unsigned long long*)work_ctr, 1ull);
+        if (left == 1 && threadIdx.x == 0)
+            s_next = atomicAdd((unsigned long long*)work_ctr, (unsigned long long)KB_PROBE_GRAB);
         uint32_t sb, chunk = 0, sl = 0, nsl = 1;
         const bool heavy = item < n_heavy;
         if (heavy) {
@@ -471,7 +476,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         const uint64_t nb = build_r ? nr : ns, np = build_r ? ns : nr;
         const bool skip = nr == 0 || ns == 0 || (!heavy && (nb > (uint64_t)CAP || np > (uint64_t)slice));
         if (!skip) {
-            const int b = find_join(pxsb, B.n, sb);
+            const int b = (int)B.sbj[sb];
             const int bits = B.J[b].bits;
             const uint64_t ob = B.J[b].out_base, oc = B.J[b].out_cap;
             const uint64_t b_lo = (uint64_t)chunk * CAP;
@@ -487,7 +492,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                                            o_sval + ob);
         }
         __syncthreads();  // the table is rebuilt by the next item; s_next is written
-        item = s_next;
+        if (left > 1) {
+            item++;
+            left--;
+        } else {
+            item = s_next;
+            left = KB_PROBE_GRAB;
+        }
         __syncthreads();
     }
 }
@@ -545,21 +556,22 @@ hipError_t launch_batch_binscan(const BatchDev& B, int side, const uint32_t* tot
     return hipGetLastError();
 }
 
-hipError_t launch_batch_scatter(const BatchDev& B, int side, uint32_t ntasks, const uint32_t* sp_key,
-                                const uint32_t* sp_val, const uint64_t* jc, const uint32_t* cur_rel,
-                                const uint64_t* start, uint32_t* okey, uint32_t* oval, uint64_t out_cap,
-                                uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err, bool big,
-                                hipStream_t st)
+hipError_t launch_batch_scatter(const BatchDev& B, int side, int cls, uint32_t ntasks, int max_b1,
+                                const uint32_t* sp_key, const uint32_t* sp_val, const uint64_t* jc,
+                                const uint32_t* cur_rel, const uint64_t* start, uint32_t* okey, uint32_t* oval,
+                                uint64_t out_cap, uint32_t* tkey, uint32_t* tval, uint64_t tmp_cap, uint64_t* err,
+                                bool big, hipStream_t st)
 {
     if (ntasks == 0) return hipSuccess;
-    const size_t lds_big = std::max(scatter_lds_bytes_big(1 << B.max_b1), g_max_lds_b / 2 + 1024);
+    if (side < 0 || side > 2 || cls < 0 || cls > 1) return hipErrorInvalidValue;
+    const size_t lds_big = std::max(scatter_lds_bytes_big(1 << max_b1), g_max_lds_b / 2 + 1024);
     if (big && lds_big <= g_max_lds_b) {
         kb_scatter<BS_BLOCK, 2 * BS_ITEMS><<<ntasks, BS_BLOCK, lds_big, st>>>(
-            B, side, sp_key, sp_val, jc, cur_rel, start, okey, oval, out_cap, tkey, tval, tmp_cap, err);
+            B, side, cls, sp_key, sp_val, jc, cur_rel, start, okey, oval, out_cap, tkey, tval, tmp_cap, err);
         return hipGetLastError();
     }
-    const size_t lds = scatter_lds_bytes(1 << B.max_b1);
-    kb_scatter<BS_BLOCK, BS_ITEMS><<<ntasks, BS_BLOCK, lds, st>>>(B, side, sp_key, sp_val, jc, cur_rel, start,
+    const size_t lds = scatter_lds_bytes(1 << max_b1);
+    kb_scatter<BS_BLOCK, BS_ITEMS><<<ntasks, BS_BLOCK, lds, st>>>(B, side, cls, sp_key, sp_val, jc, cur_rel, start,
                                                                   okey, oval, out_cap, tkey, tval, tmp_cap, err);
     return hipGetLastError();
 }
