@@ -557,6 +557,37 @@ def main() -> int:
         print(f"wrong match count {res.match_count} != {m_s}", file=sys.stderr)
         return 1
 
+    # The same join at one radix bit fewer and more than configured (the
+    # configured run above is the headline; this is the autotuned-bits report)
+    bits_sweep = None
+    if not args.host_inputs and not args.s_chunks and args.pass1_bits is None and args.s_pass1_bits is None:
+        base_bits = int(res.bits1) + int(res.bits2)
+        cand = {}
+        for b in (base_bits - 1, base_bits, base_bits + 1):
+            if b < 1 or b > 15:
+                continue
+            q = bj.default_params(r_size=n_r, s_size=n_s, inputs_on_device=1, **dict(params, radix_bits=b))
+            q.collect_phase_times = 0
+            try:
+                r = ctx.join(q, rk, rv, sk, sv)
+            except bj.BjError:
+                continue
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            reps = max(1, min(3, args.steps))
+            for _ in range(reps):
+                r = ctx.join(q, rk, rv, sk, sv)
+            torch.cuda.synchronize()
+            if r.match_count != m_s:
+                print(f"wrong match count at {b} radix bits", file=sys.stderr)
+                return 1
+            cand[b] = (time.perf_counter() - t1) / reps * 1e3
+        if cand:
+            best = min(cand, key=cand.get)
+            bits_sweep = {"configured_bits": base_bits, "ms_by_bits": {str(b): round(v, 3) for b, v in cand.items()},
+                          "best_bits": best, "best_ms": round(cand[best], 3),
+                          "best_value": (n_r + n_s) / (cand[best] * 1e-3)}
+
     probe_phases = None
     if args.probe_breakdown:
         p.probe_breakdown = 1
@@ -613,6 +644,8 @@ def main() -> int:
                        "h2d_ms_before_join": round(h2d_sum / args.steps, 4)},
         "kernels": per_kernel,
     }
+    if bits_sweep is not None:
+        out["autotuned_bits"] = bits_sweep
     if probe_phases is not None:
         out["probe_phases_ms"] = probe_phases
     ctx.close()
