@@ -222,8 +222,12 @@ int make_plan(const bj_params& p, Plan& pl)
         else if (b1 < 0 && b2 < 0) split_bits(total, b1, b2);
         if (b1 < 1 || b2 < 0) return fail(BJ_ERR_INVALID, "radix_bits smaller than the given fan-out");
     } else {
+        // smallest bits with min(R, S) / 2^bits <= 0.96 * tile: the mean sub-bucket
+        // then lies over 4 standard deviations below the tile for uniform keys at
+        // the sizes where the margin matters (16M: 11 bits, 0.466 against 0.487 ms
+        // at 12; 64M x 1.28B: 13 bits, 12.81 against 13.07 ms at 14)
         const uint64_t m = std::max<uint64_t>(1, std::min(pl.nr, pl.ns));
-        const double per = 0.6 * pl.tile;
+        const double per = 0.96 * pl.tile;
         int need = std::max(1, ceil_log2((uint64_t)std::ceil((double)m / per)));
         need = std::max(need, seg_need);
         need = std::min(need, MAX_BITS);

@@ -100,7 +100,8 @@ typedef struct bj_params {
     int32_t second_partition_size;  /* --second_partition_size: second-level fan-out
                                        P2 (power of two, 1..2048); -1 = auto           */
     int32_t radix_bits;             /* total radix bits log2(P1*P2) (1..15); -1 = from
-                                       the two fan-outs or auto                        */
+                                       the two fan-outs or auto (the fewest bits with
+                                       min(R,S)/2^bits <= 0.96 * tile)                 */
     int32_t use_bf;                 /* argv[8]: 0 = Bloom off; any supported nonzero
                                        mode = Bloom on (one GPU: the filter built from
                                        R is already complete, so every reference mode

@@ -59,7 +59,7 @@ def test_plan_config1():
     info = bj.plan(bj.default_params(r_size=16_000_000, s_size=16_000_000, partition_size=1024))
     assert info.bits1 == 10 and info.bits2 >= 1 and info.bf == 0
     per = 16_000_000 / (1 << (info.bits1 + info.bits2))
-    assert per <= 0.6 * info.tile
+    assert per <= 0.96 * info.tile and per > 0.48 * info.tile
     assert info.capacity == 16_000_000
 
 
