@@ -440,6 +440,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="join640m_bf10", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-bits-sweep", action="store_true",
+                    help="skip the joins at one radix bit fewer and more (profiled runs: one configuration)")
     ap.add_argument("--cpu-budget-s", type=float, default=60.0,
                     help="wall-clock budget of the timed CPU count-only runs")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve8: length of the arrival process")
@@ -560,7 +562,8 @@ def main() -> int:
     # The same join at one radix bit fewer and more than configured (the
     # configured run above is the headline; this is the autotuned-bits report)
     bits_sweep = None
-    if not args.host_inputs and not args.s_chunks and args.pass1_bits is None and args.s_pass1_bits is None:
+    if (not args.no_bits_sweep and not args.host_inputs and not args.s_chunks and args.pass1_bits is None
+            and args.s_pass1_bits is None):
         base_bits = int(res.bits1) + int(res.bits2)
         cand = {}
         for b in (base_bits - 1, base_bits, base_bits + 1):
