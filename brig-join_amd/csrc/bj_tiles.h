@@ -454,10 +454,12 @@ __device__ __forceinline__ void probe_item(ProbeLds<BLOCK, CAP>& L, const uint32
     if (pn == 0) return;  // an empty slice (uniform)
 
     // table geometry: 2^tb buckets, about two per build tuple when the probe
-    // side is much longer than the build side (shorter runs per probe key), else
-    // one to two build tuples per bucket (less table to clear and scan)
+    // side is at least as long as the build side (shorter runs per probe key:
+    // 5-7% off the probe of sub-buckets with 1-4 times as many probe as build
+    // tuples, against a threshold of 8 times), else one to two build tuples per
+    // bucket (less table to clear and scan)
     int tb = 0;
-    if (pn >= 8u * (uint32_t)bn) {
+    if (pn >= (uint32_t)bn) {
         while ((1 << tb) < 2 * bn && (1 << tb) < CAP) tb++;
     } else {
         while ((1 << (tb + 1)) < bn && (1 << (tb + 1)) <= CAP / 2) tb++;
