@@ -612,46 +612,48 @@ void bj_params_default(bj_params* p)
 
 int bj_create(bj_ctx** out, int device)
 {
-    if (!out) return fail(BJ_ERR_INVALID, "null ctx pointer");
-    *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-        (void)hipGetLastError();
-        return fail(BJ_ERR_NODEVICE, "no HIP device");
-    }
-    if (device < 0 || device >= ndev) return fail(BJ_ERR_NODEVICE, "device index out of range");
-    BJ_HIP(hipSetDevice(device));
-    bj_ctx* c = new bj_ctx();
-    c->device = device;
-    int cu = 0;
-    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
-        c->num_cu = cu;
-    int max_lds = 64 * 1024;
-    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
-        max_lds = 64 * 1024;
-    c->max_lds = max_lds;
-    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = kernels_init(max_lds);
-    // the scratch array of bj_join_noalloc on this device (its failure only
-    // disables that entry point, which then reports it)
-    if (e == hipSuccess) (void)noalloc_init(device);
-    if (e == hipSuccess) e = batch_kernels_init(max_lds);
-    c->ev.assign(EV_N, nullptr);
-    for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_counters, (CTR_N + 1) * sizeof(uint64_t), hipHostMallocDefault);
-    if (e != hipSuccess) {
-        bj_destroy(c);
-        return fail(BJ_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
-    }
-    c->stream = c->own_stream;
-    int rc = ensure(c->counters, CTR_N * sizeof(uint64_t));
-    if (rc == BJ_OK) rc = ensure(c->dtmp, 64);
-    if (rc != BJ_OK) {
-        bj_destroy(c);
-        return rc;
-    }
-    *out = c;
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!out) return fail(BJ_ERR_INVALID, "null ctx pointer");
+        *out = nullptr;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+            (void)hipGetLastError();
+            return fail(BJ_ERR_NODEVICE, "no HIP device");
+        }
+        if (device < 0 || device >= ndev) return fail(BJ_ERR_NODEVICE, "device index out of range");
+        BJ_HIP(hipSetDevice(device));
+        bj_ctx* c = new bj_ctx();
+        c->device = device;
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
+            c->num_cu = cu;
+        int max_lds = 64 * 1024;
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+            max_lds = 64 * 1024;
+        c->max_lds = max_lds;
+        hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = kernels_init(max_lds);
+        // the scratch array of bj_join_noalloc on this device (its failure only
+        // disables that entry point, which then reports it)
+        if (e == hipSuccess) (void)noalloc_init(device);
+        if (e == hipSuccess) e = batch_kernels_init(max_lds);
+        c->ev.assign(EV_N, nullptr);
+        for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_counters, (CTR_N + 1) * sizeof(uint64_t), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            bj_destroy(c);
+            return fail(BJ_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
+        }
+        c->stream = c->own_stream;
+        int rc = ensure(c->counters, CTR_N * sizeof(uint64_t));
+        if (rc == BJ_OK) rc = ensure(c->dtmp, 64);
+        if (rc != BJ_OK) {
+            bj_destroy(c);
+            return rc;
+        }
+        *out = c;
+        return BJ_OK;
+    });
 }
 
 int bj_destroy(bj_ctx* c)
@@ -690,25 +692,27 @@ int bj_set_stream(bj_ctx* c, void* s)
 
 int bj_plan(const bj_params* p, bj_plan_info* out)
 {
-    if (!p || !out) return fail(BJ_ERR_INVALID, "null argument");
-    std::memset(out, 0, sizeof(*out));
-    Plan pl;
-    int rc = make_plan(*p, pl);
-    if (rc) return rc;
-    uint64_t ws = 0;
-    ensure_workspace(nullptr, pl, &ws);
-    if (!p->inputs_on_device) ws += (pl.nr + pl.ns) * 8;
-    out->bits1 = pl.bits1;
-    out->bits2 = pl.bits2;
-    out->bf = pl.bf ? 1 : 0;
-    out->bf_seg_bits = pl.bf ? pl.geom.seg_bits : 0;
-    out->bf_k = pl.bf ? pl.geom.k : 0;
-    out->bf_words = pl.bf_words;
-    out->tile = pl.tile;
-    out->capacity = pl.cap;
-    out->workspace_bytes = ws;
-    out->bf_stage = pl.bf ? pl.bf_stage : 0;
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!p || !out) return fail(BJ_ERR_INVALID, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        Plan pl;
+        int rc = make_plan(*p, pl);
+        if (rc) return rc;
+        uint64_t ws = 0;
+        ensure_workspace(nullptr, pl, &ws);
+        if (!p->inputs_on_device) ws += (pl.nr + pl.ns) * 8;
+        out->bits1 = pl.bits1;
+        out->bits2 = pl.bits2;
+        out->bf = pl.bf ? 1 : 0;
+        out->bf_seg_bits = pl.bf ? pl.geom.seg_bits : 0;
+        out->bf_k = pl.bf ? pl.geom.k : 0;
+        out->bf_words = pl.bf_words;
+        out->tile = pl.tile;
+        out->capacity = pl.cap;
+        out->workspace_bytes = ws;
+        out->bf_stage = pl.bf ? pl.bf_stage : 0;
+        return BJ_OK;
+    });
 }
 
 }  // extern "C"
@@ -1139,55 +1143,61 @@ static int join_once(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const
 int bj_join(bj_ctx* c, const bj_params* p, const uint32_t* r_key, const uint32_t* r_val,
             const uint32_t* s_key, const uint32_t* s_val, bj_result* out)
 {
-    const int rc = join_once(c, p, r_key, r_val, s_key, s_val, out);
-    if (rc != BJ_OK && c && c->stream) {
-        // an error return leaves no copy of the caller's buffers and no kernel
-        // on them in flight: the caller may free them once this returns
-        if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
-        (void)hipStreamSynchronize(c->stream);
-    }
-    return rc;
+    return guarded([&]() -> int {
+        const int rc = join_once(c, p, r_key, r_val, s_key, s_val, out);
+        if (rc != BJ_OK && c && c->stream) {
+            // an error return leaves no copy of the caller's buffers and no kernel
+            // on them in flight: the caller may free them once this returns
+            if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+            (void)hipStreamSynchronize(c->stream);
+        }
+        return rc;
+    });
 }
 
 int bj_partition(bj_ctx* c, const bj_params* p, const uint32_t* key, const uint32_t* val, int64_t n,
                  uint32_t* out_key, uint32_t* out_val, uint64_t* out_start)
 {
-    if (!c || !p || !out_start || (n > 0 && (!key || !val || !out_key || !out_val)))
-        return fail(BJ_ERR_INVALID, "null argument");
-    bj_params q = *p;
-    q.r_size = n;
-    q.s_size = n;
-    q.use_bf = 0;
-    Plan pl;
-    int rc = make_plan(q, pl);
-    if (rc) return rc;
-    BJ_HIP(hipSetDevice(c->device));
-    if ((rc = ensure_workspace(c, pl))) return rc;
-    hipStream_t st = c->stream;
-    uint64_t* ctr = P<uint64_t>(c->counters);
-    tl_reset(c, false);
-    BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
-    if ((rc = run_partition(c, pl, pl.p1, key, val, pl.nr, nullptr, pl.G_r, pl.solo_r, out_key, out_val, pl.nr, out_start,
-                            BJ_K_HIST_R)))
-        return rc;
-    BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    BJ_HIP(hipStreamSynchronize(st));
-    if (c->h_counters[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(c->h_counters[CTR_ERR]));
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!c || !p || !out_start || (n > 0 && (!key || !val || !out_key || !out_val)))
+            return fail(BJ_ERR_INVALID, "null argument");
+        bj_params q = *p;
+        q.r_size = n;
+        q.s_size = n;
+        q.use_bf = 0;
+        Plan pl;
+        int rc = make_plan(q, pl);
+        if (rc) return rc;
+        BJ_HIP(hipSetDevice(c->device));
+        if ((rc = ensure_workspace(c, pl))) return rc;
+        hipStream_t st = c->stream;
+        uint64_t* ctr = P<uint64_t>(c->counters);
+        tl_reset(c, false);
+        BJ_HIP(launch_zero_u64(ctr, CTR_N, st));
+        if ((rc = run_partition(c, pl, pl.p1, key, val, pl.nr, nullptr, pl.G_r, pl.solo_r, out_key, out_val, pl.nr, out_start,
+                                BJ_K_HIST_R)))
+            return rc;
+        BJ_HIP(hipMemcpyAsync(c->h_counters, ctr, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        BJ_HIP(hipStreamSynchronize(st));
+        if (c->h_counters[CTR_ERR]) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(c->h_counters[CTR_ERR]));
+        return BJ_OK;
+    });
 }
 
 int bj_copy_rows_to_host(bj_ctx* c, const bj_result* r, uint32_t* key, uint32_t* r_val, uint32_t* s_val,
                          uint64_t n)
 {
-    if (!c || !r) return fail(BJ_ERR_INVALID, "null argument");
-    if (n > r->rows_written) return fail(BJ_ERR_INVALID, "n exceeds rows_written");
-    if (n == 0) return BJ_OK;
-    BJ_HIP(hipSetDevice(c->device));
-    if (key) BJ_HIP(hipMemcpyAsync(key, r->key, n * 4, hipMemcpyDeviceToHost, c->stream));
-    if (r_val) BJ_HIP(hipMemcpyAsync(r_val, r->r_val, n * 4, hipMemcpyDeviceToHost, c->stream));
-    if (s_val) BJ_HIP(hipMemcpyAsync(s_val, r->s_val, n * 4, hipMemcpyDeviceToHost, c->stream));
-    BJ_HIP(hipStreamSynchronize(c->stream));
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!c || !r) return fail(BJ_ERR_INVALID, "null argument");
+        if (n > r->rows_written) return fail(BJ_ERR_INVALID, "n exceeds rows_written");
+        if (n == 0) return BJ_OK;
+        BJ_HIP(hipSetDevice(c->device));
+        if (key) BJ_HIP(hipMemcpyAsync(key, r->key, n * 4, hipMemcpyDeviceToHost, c->stream));
+        if (r_val) BJ_HIP(hipMemcpyAsync(r_val, r->r_val, n * 4, hipMemcpyDeviceToHost, c->stream));
+        if (s_val) BJ_HIP(hipMemcpyAsync(s_val, r->s_val, n * 4, hipMemcpyDeviceToHost, c->stream));
+        BJ_HIP(hipStreamSynchronize(c->stream));
+        return BJ_OK;
+    });
 }
 
 int bj_gen_build(bj_ctx* c, uint64_t seed, uint64_t n_r, uint32_t* d_key, uint32_t* d_val)

@@ -513,18 +513,20 @@ void bj_join_spec_default(bj_join_spec* s)
 
 int bj_batch_plan(const bj_ctx* c, int n, const bj_join_spec* specs, bj_join_out* cfg, uint64_t* workspace_bytes)
 {
-    Layout L;
-    int rc = make_layout(c, n, specs, L);
-    if (rc) return rc;
-    if (cfg) {
-        for (int i = 0; i < n; i++) {
-            bj_join_out& o = cfg[L.order[i]];
-            std::memset(&o, 0, sizeof(o));
-            fill_cfg(L.cfg[i], o);
+    return guarded([&]() -> int {
+        Layout L;
+        int rc = make_layout(c, n, specs, L);
+        if (rc) return rc;
+        if (cfg) {
+            for (int i = 0; i < n; i++) {
+                bj_join_out& o = cfg[L.order[i]];
+                std::memset(&o, 0, sizeof(o));
+                fill_cfg(L.cfg[i], o);
+            }
         }
-    }
-    if (workspace_bytes) batch_workspace(nullptr, L, n, workspace_bytes);
-    return BJ_OK;
+        if (workspace_bytes) batch_workspace(nullptr, L, n, workspace_bytes);
+        return BJ_OK;
+    });
 }
 
 int bj_batch_destroy(bj_batch* b)
@@ -551,104 +553,108 @@ int bj_batch_destroy(bj_batch* b)
 
 int bj_batch_create(bj_ctx* c, int n, const bj_join_spec* specs, int use_graph, bj_batch** out)
 {
-    if (!c || !out) return fail(BJ_ERR_INVALID, "null argument");
-    *out = nullptr;
-    for (int i = 0; i < n && specs; i++) {
-        const bj_join_spec& s = specs[i];
-        if ((s.r_size > 0 && (!s.r_key || !s.r_val)) || (s.s_size > 0 && (!s.s_key || !s.s_val)))
-            return fail(BJ_ERR_INVALID, "join " + std::to_string(i) + ": null relation pointer");
-    }
-    bj_batch* b = new bj_batch();
-    b->c = c;
-    b->n = n;
-    int rc = make_layout(c, n, specs, b->L);
-    if (rc) {
-        delete b;
-        return rc;
-    }
-    hipError_t e = hipSetDevice(c->device);
-    if (e == hipSuccess) e = hipEventCreate(&b->e0);
-    if (e == hipSuccess) e = hipEventCreate(&b->e1);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_ctr, ctr_words(n) * 8, hipHostMallocDefault);
-    if (e != hipSuccess) {
-        bj_batch_destroy(b);
-        return fail(BJ_ERR_HIP, std::string("batch setup: ") + hipGetErrorString(e));
-    }
-    if ((rc = batch_workspace(b, b->L, n, nullptr))) {
-        bj_batch_destroy(b);
-        return rc;
-    }
-    hipStream_t st = c->stream;
-    e = hipMemcpyAsync(b->dJ.p, b->L.J.data(), sizeof(BJoin) * n, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->dpx.p, b->L.px.data(), b->L.px.size() * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
-        bj_batch_destroy(b);
-        return fail(BJ_ERR_HIP, std::string("batch descriptors: ") + hipGetErrorString(e));
-    }
-    if (use_graph) {
-        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    return guarded([&]() -> int {
+        if (!c || !out) return fail(BJ_ERR_INVALID, "null argument");
+        *out = nullptr;
+        for (int i = 0; i < n && specs; i++) {
+            const bj_join_spec& s = specs[i];
+            if ((s.r_size > 0 && (!s.r_key || !s.r_val)) || (s.s_size > 0 && (!s.s_key || !s.s_val)))
+                return fail(BJ_ERR_INVALID, "join " + std::to_string(i) + ": null relation pointer");
+        }
+        bj_batch* b = new bj_batch();
+        b->c = c;
+        b->n = n;
+        int rc = make_layout(c, n, specs, b->L);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        hipError_t e = hipSetDevice(c->device);
+        if (e == hipSuccess) e = hipEventCreate(&b->e0);
+        if (e == hipSuccess) e = hipEventCreate(&b->e1);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_ctr, ctr_words(n) * 8, hipHostMallocDefault);
         if (e != hipSuccess) {
             bj_batch_destroy(b);
-            return fail(BJ_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+            return fail(BJ_ERR_HIP, std::string("batch setup: ") + hipGetErrorString(e));
         }
-        rc = enqueue(b, st);
-        hipGraph_t g = nullptr;
-        e = hipStreamEndCapture(st, &g);
-        b->graph = g;
-        if (rc) {
+        if ((rc = batch_workspace(b, b->L, n, nullptr))) {
             bj_batch_destroy(b);
             return rc;
         }
-        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        hipStream_t st = c->stream;
+        e = hipMemcpyAsync(b->dJ.p, b->L.J.data(), sizeof(BJoin) * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->dpx.p, b->L.px.data(), b->L.px.size() * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
             bj_batch_destroy(b);
-            return fail(BJ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            return fail(BJ_ERR_HIP, std::string("batch descriptors: ") + hipGetErrorString(e));
         }
-    }
-    *out = b;
-    return BJ_OK;
+        if (use_graph) {
+            e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) {
+                bj_batch_destroy(b);
+                return fail(BJ_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+            }
+            rc = enqueue(b, st);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(st, &g);
+            b->graph = g;
+            if (rc) {
+                bj_batch_destroy(b);
+                return rc;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                bj_batch_destroy(b);
+                return fail(BJ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            }
+        }
+        *out = b;
+        return BJ_OK;
+    });
 }
 
 int bj_batch_run(bj_batch* b, bj_join_out* outs, double* device_ms)
 {
-    if (!b || (b->n > 0 && !outs)) return fail(BJ_ERR_INVALID, "null argument");
-    bj_ctx* c = b->c;
-    BJ_HIP(hipSetDevice(c->device));
-    hipStream_t st = c->stream;
-    BJ_HIP(hipEventRecord(b->e0, st));
-    if (b->exec) {
-        BJ_HIP(hipGraphLaunch(b->exec, st));
-    } else {
-        int rc = enqueue(b, st);
-        if (rc) return rc;
-    }
-    BJ_HIP(hipEventRecord(b->e1, st));
-    BJ_HIP(hipMemcpyAsync(b->h_ctr, b->ctr.p, ctr_words(b->n) * 8, hipMemcpyDeviceToHost, st));
-    BJ_HIP(hipStreamSynchronize(st));
-    const uint64_t err = b->h_ctr[(size_t)b->n * JC_N + 2];
-    if (err) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(err));
-    if (device_ms) {
-        float ms = 0.f;
-        BJ_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
-        *device_ms = ms;
-    }
-    for (int i = 0; i < b->n; i++) {
-        const BJoin& J = b->L.J[i];
-        bj_join_out& o = outs[b->L.order[i]];
-        std::memset(&o, 0, sizeof(o));
-        fill_cfg(b->L.cfg[i], o);
-        const uint64_t cnt = b->h_ctr[(size_t)i * JC_N + JC_OUT];
-        o.match_count = cnt;
-        o.rows_written = std::min(cnt, J.out_cap);
-        o.truncated = cnt > J.out_cap ? 1 : 0;
-        o.s_survivors = J.bf ? b->h_ctr[(size_t)i * JC_N + JC_SPRIME] : J.ns;
-        o.key = P<uint32_t>(b->out_key) + J.out_base;
-        o.r_val = P<uint32_t>(b->out_rval) + J.out_base;
-        o.s_val = P<uint32_t>(b->out_sval) + J.out_base;
-    }
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!b || (b->n > 0 && !outs)) return fail(BJ_ERR_INVALID, "null argument");
+        bj_ctx* c = b->c;
+        BJ_HIP(hipSetDevice(c->device));
+        hipStream_t st = c->stream;
+        BJ_HIP(hipEventRecord(b->e0, st));
+        if (b->exec) {
+            BJ_HIP(hipGraphLaunch(b->exec, st));
+        } else {
+            int rc = enqueue(b, st);
+            if (rc) return rc;
+        }
+        BJ_HIP(hipEventRecord(b->e1, st));
+        BJ_HIP(hipMemcpyAsync(b->h_ctr, b->ctr.p, ctr_words(b->n) * 8, hipMemcpyDeviceToHost, st));
+        BJ_HIP(hipStreamSynchronize(st));
+        const uint64_t err = b->h_ctr[(size_t)b->n * JC_N + 2];
+        if (err) return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(err));
+        if (device_ms) {
+            float ms = 0.f;
+            BJ_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+            *device_ms = ms;
+        }
+        for (int i = 0; i < b->n; i++) {
+            const BJoin& J = b->L.J[i];
+            bj_join_out& o = outs[b->L.order[i]];
+            std::memset(&o, 0, sizeof(o));
+            fill_cfg(b->L.cfg[i], o);
+            const uint64_t cnt = b->h_ctr[(size_t)i * JC_N + JC_OUT];
+            o.match_count = cnt;
+            o.rows_written = std::min(cnt, J.out_cap);
+            o.truncated = cnt > J.out_cap ? 1 : 0;
+            o.s_survivors = J.bf ? b->h_ctr[(size_t)i * JC_N + JC_SPRIME] : J.ns;
+            o.key = P<uint32_t>(b->out_key) + J.out_base;
+            o.r_val = P<uint32_t>(b->out_rval) + J.out_base;
+            o.s_val = P<uint32_t>(b->out_sval) + J.out_base;
+        }
+        return BJ_OK;
+    });
 }
 
 // ------------------------------------------------------------------ autotuner
@@ -663,17 +669,19 @@ int bj_tune_get(const bj_ctx* c, bj_tune_entry* out, int cap, int* n)
 
 int bj_tune_set(bj_ctx* c, const bj_tune_entry* in, int n)
 {
-    if (!c || (n > 0 && !in) || n < 0) return fail(BJ_ERR_INVALID, "bad argument");
-    for (int i = 0; i < n; i++) {
-        const bj_tune_entry& e = in[i];
-        if (e.kind == 0 && (e.radix_bits < 1 || e.radix_bits > B_MAX_BITS || (e.tile != 4096 && e.tile != 8192)))
-            return fail(BJ_ERR_INVALID, "tune entry " + std::to_string(i) + ": bad radix bits or tile");
-        if (e.kind == 1 && (e.bf_bits_per_key < 0.0 || e.bf_bits_per_key > 64.0))
-            return fail(BJ_ERR_INVALID, "tune entry " + std::to_string(i) + ": bad filter bits per key");
-        if (e.kind != 0 && e.kind != 1) return fail(BJ_ERR_INVALID, "tune entry kind must be 0 or 1");
-    }
-    c->tune.assign(in, in + n);
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!c || (n > 0 && !in) || n < 0) return fail(BJ_ERR_INVALID, "bad argument");
+        for (int i = 0; i < n; i++) {
+            const bj_tune_entry& e = in[i];
+            if (e.kind == 0 && (e.radix_bits < 1 || e.radix_bits > B_MAX_BITS || (e.tile != 4096 && e.tile != 8192)))
+                return fail(BJ_ERR_INVALID, "tune entry " + std::to_string(i) + ": bad radix bits or tile");
+            if (e.kind == 1 && (e.bf_bits_per_key < 0.0 || e.bf_bits_per_key > 64.0))
+                return fail(BJ_ERR_INVALID, "tune entry " + std::to_string(i) + ": bad filter bits per key");
+            if (e.kind != 0 && e.kind != 1) return fail(BJ_ERR_INVALID, "tune entry kind must be 0 or 1");
+        }
+        c->tune.assign(in, in + n);
+        return BJ_OK;
+    });
 }
 
 }  // extern "C"
@@ -750,92 +758,94 @@ int time_config(bj_ctx* c, TuneData& d, int bits, int tile, double bpk, double& 
 
 extern "C" int bj_autotune(bj_ctx* c, int n, const bj_join_spec* specs, int* entries_tuned)
 {
-    if (!c || (n > 0 && !specs) || n < 0) return fail(BJ_ERR_INVALID, "bad argument");
-    if (entries_tuned) *entries_tuned = 0;
-    BJ_HIP(hipSetDevice(c->device));
-    std::map<int, bool> lgs;
-    std::map<std::pair<int, int>, bool> bfs;
-    for (int i = 0; i < n; i++) {
-        const bj_join_spec& s = specs[i];
-        if (s.r_size <= 0 || s.s_size <= 0) continue;
-        const uint64_t nr = (uint64_t)s.r_size, ns = (uint64_t)s.s_size;
-        if (s.radix_bits <= 0 || s.tile == 0) {
+    return guarded([&]() -> int {
+        if (!c || (n > 0 && !specs) || n < 0) return fail(BJ_ERR_INVALID, "bad argument");
+        if (entries_tuned) *entries_tuned = 0;
+        BJ_HIP(hipSetDevice(c->device));
+        std::map<int, bool> lgs;
+        std::map<std::pair<int, int>, bool> bfs;
+        for (int i = 0; i < n; i++) {
+            const bj_join_spec& s = specs[i];
+            if (s.r_size <= 0 || s.s_size <= 0) continue;
+            const uint64_t nr = (uint64_t)s.r_size, ns = (uint64_t)s.s_size;
+            if (s.radix_bits <= 0 || s.tile == 0) {
+                const int lg = lg_class(std::min(nr, ns));
+                if (!find_tune(c, 0, lg, 0, 0)) lgs[lg] = true;
+            }
+            if (s.bf_bits_per_key < 0.0) {
+                const int sr = sr_class(nr, ns), sel = sel_class(s.match_hint), fcls = filter_class(nr);
+                if (!find_tune(c, 1, fcls, sr, sel)) bfs[{fcls * 16 + sr + 3, sel}] = true;
+            }
+        }
+        TuneData d;
+        int tuned = 0;
+        int rc = BJ_OK;
+        // radix bits and tile per build-size class, filter off
+        for (const auto& kv : lgs) {
+            const int lg = kv.first;
+            const uint64_t nr = 1ull << lg, ns = 2ull << lg;
+            const int m = (int)std::max<uint64_t>(1, std::min<uint64_t>(32, (1ull << 23) >> lg));
+            if ((rc = gen_tune_data(c, d, m, nr, ns, 0.1))) break;
+            bj_tune_entry best{};
+            best.kind = 0;
+            best.lg = lg;
+            best.ms = 1e30;
+            for (int tile : {8192, 4096}) {
+                const int b0 = rule_bits(nr, tile);
+                for (int bits = std::max(1, b0 - 1); bits <= std::min(B_MAX_BITS, b0 + 1); bits++) {
+                    double ms = 0.0;
+                    if ((rc = time_config(c, d, bits, tile, 0.0, ms))) break;
+                    if (ms < best.ms) {
+                        best.ms = ms;
+                        best.radix_bits = bits;
+                        best.tile = tile;
+                    }
+                }
+                if (rc) break;
+            }
+            if (rc) break;
+            c->tune.push_back(best);
+            tuned++;
+        }
+        // filter bits per key per (filter class, |S|/|R|, selectivity) class: at
+        // |R| = 2^20 for filters that stay in the L2, at 2^23 (two joins) for those
+        // probed after a first pass of S
+        for (const auto& kv : bfs) {
+            if (rc) break;
+            const int fcls = kv.first.first / 16, sr = kv.first.first % 16 - 3, sel = kv.first.second;
+            const uint64_t nr = fcls ? (1ull << 23) : (1ull << 20);
+            const uint64_t ns = sr >= 0 ? (nr << sr) : (nr >> -sr);
+            if ((rc = gen_tune_data(c, d, fcls ? 2 : 8, nr, ns, SEL_REP[sel]))) break;
             const int lg = lg_class(std::min(nr, ns));
-            if (!find_tune(c, 0, lg, 0, 0)) lgs[lg] = true;
-        }
-        if (s.bf_bits_per_key < 0.0) {
-            const int sr = sr_class(nr, ns), sel = sel_class(s.match_hint), fcls = filter_class(nr);
-            if (!find_tune(c, 1, fcls, sr, sel)) bfs[{fcls * 16 + sr + 3, sel}] = true;
-        }
-    }
-    TuneData d;
-    int tuned = 0;
-    int rc = BJ_OK;
-    // radix bits and tile per build-size class, filter off
-    for (const auto& kv : lgs) {
-        const int lg = kv.first;
-        const uint64_t nr = 1ull << lg, ns = 2ull << lg;
-        const int m = (int)std::max<uint64_t>(1, std::min<uint64_t>(32, (1ull << 23) >> lg));
-        if ((rc = gen_tune_data(c, d, m, nr, ns, 0.1))) break;
-        bj_tune_entry best{};
-        best.kind = 0;
-        best.lg = lg;
-        best.ms = 1e30;
-        for (int tile : {8192, 4096}) {
-            const int b0 = rule_bits(nr, tile);
-            for (int bits = std::max(1, b0 - 1); bits <= std::min(B_MAX_BITS, b0 + 1); bits++) {
+            const bj_tune_entry* t0 = find_tune(c, 0, lg, 0, 0);
+            const int tile = t0 ? t0->tile : 8192;
+            const int bits = t0 ? t0->radix_bits : rule_bits(std::min(nr, ns), tile);
+            bj_tune_entry best{};
+            best.kind = 1;
+            best.lg = fcls;
+            best.sr = sr;
+            best.sel = sel;
+            best.ms = 1e30;
+            // a filter has to beat the join without one by 3%: near-ties otherwise
+            // flip between runs, and the filter is the riskier choice for the
+            // larger joins of the class
+            for (double bpk : {0.0, 6.0, 10.0, 14.0}) {
                 double ms = 0.0;
-                if ((rc = time_config(c, d, bits, tile, 0.0, ms))) break;
-                if (ms < best.ms) {
+                if ((rc = time_config(c, d, bits, tile, bpk, ms))) break;
+                if (bpk == 0.0 ? ms < best.ms : ms < best.ms * 0.97) {
                     best.ms = ms;
-                    best.radix_bits = bits;
-                    best.tile = tile;
+                    best.bf_bits_per_key = bpk;
                 }
             }
             if (rc) break;
+            c->tune.push_back(best);
+            tuned++;
         }
-        if (rc) break;
-        c->tune.push_back(best);
-        tuned++;
-    }
-    // filter bits per key per (filter class, |S|/|R|, selectivity) class: at
-    // |R| = 2^20 for filters that stay in the L2, at 2^23 (two joins) for those
-    // probed after a first pass of S
-    for (const auto& kv : bfs) {
-        if (rc) break;
-        const int fcls = kv.first.first / 16, sr = kv.first.first % 16 - 3, sel = kv.first.second;
-        const uint64_t nr = fcls ? (1ull << 23) : (1ull << 20);
-        const uint64_t ns = sr >= 0 ? (nr << sr) : (nr >> -sr);
-        if ((rc = gen_tune_data(c, d, fcls ? 2 : 8, nr, ns, SEL_REP[sel]))) break;
-        const int lg = lg_class(std::min(nr, ns));
-        const bj_tune_entry* t0 = find_tune(c, 0, lg, 0, 0);
-        const int tile = t0 ? t0->tile : 8192;
-        const int bits = t0 ? t0->radix_bits : rule_bits(std::min(nr, ns), tile);
-        bj_tune_entry best{};
-        best.kind = 1;
-        best.lg = fcls;
-        best.sr = sr;
-        best.sel = sel;
-        best.ms = 1e30;
-        // a filter has to beat the join without one by 3%: near-ties otherwise
-        // flip between runs, and the filter is the riskier choice for the
-        // larger joins of the class
-        for (double bpk : {0.0, 6.0, 10.0, 14.0}) {
-            double ms = 0.0;
-            if ((rc = time_config(c, d, bits, tile, bpk, ms))) break;
-            if (bpk == 0.0 ? ms < best.ms : ms < best.ms * 0.97) {
-                best.ms = ms;
-                best.bf_bits_per_key = bpk;
-            }
-        }
-        if (rc) break;
-        c->tune.push_back(best);
-        tuned++;
-    }
-    release(d.rk);
-    release(d.rv);
-    release(d.sk);
-    release(d.sv);
-    if (entries_tuned) *entries_tuned = tuned;
-    return rc;
+        release(d.rk);
+        release(d.rv);
+        release(d.sk);
+        release(d.sv);
+        if (entries_tuned) *entries_tuned = tuned;
+        return rc;
+    });
 }

@@ -80,49 +80,53 @@ extern "C" {
 int bj_gen_refdist_host(int64_t r_size, int64_t s_size, double answer_pct, uint64_t seed, double max_pct,
                         uint32_t* r_key, uint32_t* s_key, int64_t* matches)
 {
-    RefDist d;
-    int rc = refdist_plan(r_size, s_size, answer_pct, max_pct, d);
-    if (rc) return rc;
-    if ((r_size && !r_key) || (s_size && !s_key)) return fail(BJ_ERR_INVALID, "null output array");
-    const uint64_t R = (uint64_t)r_size, S = (uint64_t)s_size;
-    const uint64_t r_only = 0x7FFFFFFFull - d.k;
-    for (uint64_t i = 0; i < R; i++) r_key[i] = (uint32_t)(i < d.k ? i : d.k + (i - d.k) % r_only);
-    fisher_yates(r_key, R, seed);
-    const uint64_t s_only = 0x7FFFFFFFull - R;
-    uint64_t mc = 0, sj = 0, p = 0;
-    auto fill = [&](uint64_t m, uint64_t rest) {
-        for (uint64_t c = 0; c < m; c++) s_key[p++] = (uint32_t)(mc++);
-        for (uint64_t c = 0; c < rest; c++) s_key[p++] = (uint32_t)(R + (sj++) % s_only);
-    };
-    fill(d.m_hot, d.hot_n - d.m_hot);
-    fill(d.m_cold, d.cold_n - d.m_cold);
-    if (max_pct >= 0.0) {
-        fisher_yates(s_key, d.hot_n, seed ^ SEED_S);
-        fisher_yates(s_key + d.hot_n, d.cold_n, seed ^ SEED_COLD);
-    } else {
-        fisher_yates(s_key, S, seed ^ SEED_S);
-    }
-    if (matches) *matches = (int64_t)d.k;
-    return BJ_OK;
+    return guarded([&]() -> int {
+        RefDist d;
+        int rc = refdist_plan(r_size, s_size, answer_pct, max_pct, d);
+        if (rc) return rc;
+        if ((r_size && !r_key) || (s_size && !s_key)) return fail(BJ_ERR_INVALID, "null output array");
+        const uint64_t R = (uint64_t)r_size, S = (uint64_t)s_size;
+        const uint64_t r_only = 0x7FFFFFFFull - d.k;
+        for (uint64_t i = 0; i < R; i++) r_key[i] = (uint32_t)(i < d.k ? i : d.k + (i - d.k) % r_only);
+        fisher_yates(r_key, R, seed);
+        const uint64_t s_only = 0x7FFFFFFFull - R;
+        uint64_t mc = 0, sj = 0, p = 0;
+        auto fill = [&](uint64_t m, uint64_t rest) {
+            for (uint64_t c = 0; c < m; c++) s_key[p++] = (uint32_t)(mc++);
+            for (uint64_t c = 0; c < rest; c++) s_key[p++] = (uint32_t)(R + (sj++) % s_only);
+        };
+        fill(d.m_hot, d.hot_n - d.m_hot);
+        fill(d.m_cold, d.cold_n - d.m_cold);
+        if (max_pct >= 0.0) {
+            fisher_yates(s_key, d.hot_n, seed ^ SEED_S);
+            fisher_yates(s_key + d.hot_n, d.cold_n, seed ^ SEED_COLD);
+        } else {
+            fisher_yates(s_key, S, seed ^ SEED_S);
+        }
+        if (matches) *matches = (int64_t)d.k;
+        return BJ_OK;
+    });
 }
 
 int bj_gen_refdist(bj_ctx* c, int64_t r_size, int64_t s_size, double answer_pct, uint64_t seed, double max_pct,
                    uint32_t* d_r_key, uint32_t* d_s_key, int64_t* matches)
 {
-    if (!c) return fail(BJ_ERR_INVALID, "null ctx");
-    RefDist d;
-    int rc = refdist_plan(r_size, s_size, answer_pct, max_pct, d);
-    if (rc) return rc;
-    if ((r_size && !d_r_key) || (s_size && !d_s_key)) return fail(BJ_ERR_INVALID, "null output array");
-    BJ_HIP(hipSetDevice(c->device));
-    const uint64_t R = (uint64_t)r_size;
-    BJ_HIP(launch_gen_refdist_r(d_r_key, R, d.k, perm_keys(seed), c->stream));
-    BJ_HIP(launch_gen_refdist_s(d_s_key, d.hot_n, d.m_hot, 0, 0, R, perm_keys(seed ^ SEED_S), c->stream));
-    BJ_HIP(launch_gen_refdist_s(d_s_key + d.hot_n, d.cold_n, d.m_cold, d.m_hot, d.hot_n - d.m_hot, R,
-                                perm_keys(seed ^ SEED_COLD), c->stream));
-    BJ_HIP(hipStreamSynchronize(c->stream));
-    if (matches) *matches = (int64_t)d.k;
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!c) return fail(BJ_ERR_INVALID, "null ctx");
+        RefDist d;
+        int rc = refdist_plan(r_size, s_size, answer_pct, max_pct, d);
+        if (rc) return rc;
+        if ((r_size && !d_r_key) || (s_size && !d_s_key)) return fail(BJ_ERR_INVALID, "null output array");
+        BJ_HIP(hipSetDevice(c->device));
+        const uint64_t R = (uint64_t)r_size;
+        BJ_HIP(launch_gen_refdist_r(d_r_key, R, d.k, perm_keys(seed), c->stream));
+        BJ_HIP(launch_gen_refdist_s(d_s_key, d.hot_n, d.m_hot, 0, 0, R, perm_keys(seed ^ SEED_S), c->stream));
+        BJ_HIP(launch_gen_refdist_s(d_s_key + d.hot_n, d.cold_n, d.m_cold, d.m_hot, d.hot_n - d.m_hot, R,
+                                    perm_keys(seed ^ SEED_COLD), c->stream));
+        BJ_HIP(hipStreamSynchronize(c->stream));
+        if (matches) *matches = (int64_t)d.k;
+        return BJ_OK;
+    });
 }
 
 }  // extern "C"

@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <exception>
 #include <functional>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -16,6 +18,23 @@ namespace bj {
 
 // Records the detail for bj_last_error and returns code.
 int fail(int code, const std::string& msg);
+
+// Runs the body of an entry point: a C++ exception (a host allocation that
+// fails, a thread that cannot start) becomes an error code instead of crossing
+// the C ABI.
+template <class F>
+int guarded(F&& body) noexcept
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(BJ_ERR_NOMEM, "host memory allocation failed");
+    } catch (const std::exception& e) {
+        return fail(BJ_ERR_INTERNAL, std::string("host exception: ") + e.what());
+    } catch (...) {
+        return fail(BJ_ERR_INTERNAL, "host exception");
+    }
+}
 
 #define BJ_HIP(expr)                                                                         \
     do {                                                                                     \

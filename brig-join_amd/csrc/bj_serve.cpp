@@ -325,121 +325,125 @@ int bj_server_destroy(bj_server* s)
 
 int bj_server_create(int device, const bj_server_opts* o, bj_server** out)
 {
-    if (!out) return fail(BJ_ERR_INVALID, "null argument");
-    *out = nullptr;
-    bj_server_opts opts;
-    bj_server_opts_default(&opts);
-    if (o) opts = *o;
-    if (opts.slots < 1 || opts.slots > SRV_MAX_SLOTS) return fail(BJ_ERR_INVALID, "slots must be in [1, 8]");
-    if (opts.radix_bits == 0 || opts.radix_bits > 15) return fail(BJ_ERR_INVALID, "radix_bits must be -1 or 1..15");
-    if (opts.use_bf && !(opts.bf_bits_per_key > 0.0)) return fail(BJ_ERR_INVALID, "bf_bits_per_key must be > 0");
-    bj_ctx* main = nullptr;
-    int rc = bj_create(&main, device);
-    if (rc) return rc;
-    bj_server* s = new bj_server();
-    s->device = device;
-    s->opts = opts;
-    s->main = main;
-    s->t_origin = std::chrono::steady_clock::now();
-    uint64_t arena = opts.arena_bytes;
-    if (arena == 0) {
-        size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
-            bj_server_destroy(s);
-            return fail(BJ_ERR_HIP, "hipMemGetInfo failed");
+    return guarded([&]() -> int {
+        if (!out) return fail(BJ_ERR_INVALID, "null argument");
+        *out = nullptr;
+        bj_server_opts opts;
+        bj_server_opts_default(&opts);
+        if (o) opts = *o;
+        if (opts.slots < 1 || opts.slots > SRV_MAX_SLOTS) return fail(BJ_ERR_INVALID, "slots must be in [1, 8]");
+        if (opts.radix_bits == 0 || opts.radix_bits > 15) return fail(BJ_ERR_INVALID, "radix_bits must be -1 or 1..15");
+        if (opts.use_bf && !(opts.bf_bits_per_key > 0.0)) return fail(BJ_ERR_INVALID, "bf_bits_per_key must be > 0");
+        bj_ctx* main = nullptr;
+        int rc = bj_create(&main, device);
+        if (rc) return rc;
+        bj_server* s = new bj_server();
+        s->device = device;
+        s->opts = opts;
+        s->main = main;
+        s->t_origin = std::chrono::steady_clock::now();
+        uint64_t arena = opts.arena_bytes;
+        if (arena == 0) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+                bj_server_destroy(s);
+                return fail(BJ_ERR_HIP, "hipMemGetInfo failed");
+            }
+            arena = fr / 2;
         }
-        arena = fr / 2;
-    }
-    arena = align_up(arena);
-    if ((rc = ensure(s->arena, arena))) {
-        bj_server_destroy(s);
-        return rc;
-    }
-    s->alloc.reset(arena);
-    s->slots.resize(opts.slots);
-    for (Slot& sl : s->slots) {
-        if ((rc = bj_create(&sl.c, device))) {
+        arena = align_up(arena);
+        if ((rc = ensure(s->arena, arena))) {
             bj_server_destroy(s);
             return rc;
         }
-        release(sl.c->counters);  // a request brings its counters from the arena
-        hipError_t e = hipEventCreate(&sl.t0);
-        if (e == hipSuccess) e = hipEventCreate(&sl.t1);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_ctr, CTR_N * sizeof(uint64_t), hipHostMallocDefault);
-        if (e != hipSuccess) {
-            bj_server_destroy(s);
-            return fail(BJ_ERR_HIP, std::string("server slot: ") + hipGetErrorString(e));
+        s->alloc.reset(arena);
+        s->slots.resize(opts.slots);
+        for (Slot& sl : s->slots) {
+            if ((rc = bj_create(&sl.c, device))) {
+                bj_server_destroy(s);
+                return rc;
+            }
+            release(sl.c->counters);  // a request brings its counters from the arena
+            hipError_t e = hipEventCreate(&sl.t0);
+            if (e == hipSuccess) e = hipEventCreate(&sl.t1);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_ctr, CTR_N * sizeof(uint64_t), hipHostMallocDefault);
+            if (e != hipSuccess) {
+                bj_server_destroy(s);
+                return fail(BJ_ERR_HIP, std::string("server slot: ") + hipGetErrorString(e));
+            }
         }
-    }
-    s->sched = std::thread(scheduler, s);
-    *out = s;
-    return BJ_OK;
+        s->sched = std::thread(scheduler, s);
+        *out = s;
+        return BJ_OK;
+    });
 }
 
 int bj_server_add_relation(bj_server* s, const uint32_t* r_key, const uint32_t* r_val, int64_t n_r,
                            int32_t* rel_id)
 {
-    if (!s || !rel_id || n_r < 0 || (n_r > 0 && (!r_key || !r_val))) return fail(BJ_ERR_INVALID, "bad argument");
-    bj_params p;
-    bj_params_default(&p);
-    p.r_size = n_r;
-    p.s_size = n_r;
-    p.radix_bits = s->opts.radix_bits;
-    p.use_bf = s->opts.use_bf && n_r > 0 ? 1 : 0;
-    p.bf_bits_per_key = s->opts.bf_bits_per_key;
-    p.num_hashes = s->opts.num_hashes;
-    p.inputs_on_device = 1;
-    Relation* R = new Relation();
-    R->nr = (uint64_t)n_r;
-    int rc = make_plan(p, R->plan);
-    if (rc) {
-        delete R;
-        return rc;
-    }
-    Plan pl = R->plan;
-    pl.ns = 0;  // R side only
-    pl.G_s = 1;
-    bj_ctx* c = s->main;
-    hipError_t e = hipSetDevice(s->device);
-    if (e != hipSuccess) {
-        delete R;
-        return fail(BJ_ERR_HIP, "hipSetDevice failed");
-    }
-    if ((rc = ensure_workspace(c, pl))) {
-        delete R;
-        return rc;
-    }
-    tl_reset(c, false);
-    e = launch_zero_u64(P<uint64_t>(c->counters), CTR_N, c->stream);
-    if (e != hipSuccess) {
-        delete R;
-        return fail(BJ_ERR_HIP, std::string("relation counters: ") + hipGetErrorString(e));
-    }
-    if ((rc = enqueue_r(c, pl, r_key, r_val))) {
-        delete R;
-        return rc;
-    }
-    e = hipMemcpyAsync(c->h_counters, c->counters.p, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-        delete R;
-        return fail(BJ_ERR_HIP, std::string("relation partition: ") + hipGetErrorString(e));
-    }
-    if (c->h_counters[CTR_ERR]) {
-        delete R;
-        return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(c->h_counters[CTR_ERR]));
-    }
-    // the partitioned relation moves from the context into the server
-    std::swap(R->key, c->r_key);
-    std::swap(R->val, c->r_val);
-    std::swap(R->start, c->r_start);
-    if (R->plan.bf) std::swap(R->filter, c->filter);
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->rel_bytes += R->key.bytes + R->val.bytes + R->start.bytes + R->filter.bytes;
-    s->rels.push_back(R);
-    *rel_id = (int32_t)s->rels.size() - 1;
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!s || !rel_id || n_r < 0 || (n_r > 0 && (!r_key || !r_val))) return fail(BJ_ERR_INVALID, "bad argument");
+        bj_params p;
+        bj_params_default(&p);
+        p.r_size = n_r;
+        p.s_size = n_r;
+        p.radix_bits = s->opts.radix_bits;
+        p.use_bf = s->opts.use_bf && n_r > 0 ? 1 : 0;
+        p.bf_bits_per_key = s->opts.bf_bits_per_key;
+        p.num_hashes = s->opts.num_hashes;
+        p.inputs_on_device = 1;
+        Relation* R = new Relation();
+        R->nr = (uint64_t)n_r;
+        int rc = make_plan(p, R->plan);
+        if (rc) {
+            delete R;
+            return rc;
+        }
+        Plan pl = R->plan;
+        pl.ns = 0;  // R side only
+        pl.G_s = 1;
+        bj_ctx* c = s->main;
+        hipError_t e = hipSetDevice(s->device);
+        if (e != hipSuccess) {
+            delete R;
+            return fail(BJ_ERR_HIP, "hipSetDevice failed");
+        }
+        if ((rc = ensure_workspace(c, pl))) {
+            delete R;
+            return rc;
+        }
+        tl_reset(c, false);
+        e = launch_zero_u64(P<uint64_t>(c->counters), CTR_N, c->stream);
+        if (e != hipSuccess) {
+            delete R;
+            return fail(BJ_ERR_HIP, std::string("relation counters: ") + hipGetErrorString(e));
+        }
+        if ((rc = enqueue_r(c, pl, r_key, r_val))) {
+            delete R;
+            return rc;
+        }
+        e = hipMemcpyAsync(c->h_counters, c->counters.p, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            delete R;
+            return fail(BJ_ERR_HIP, std::string("relation partition: ") + hipGetErrorString(e));
+        }
+        if (c->h_counters[CTR_ERR]) {
+            delete R;
+            return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(c->h_counters[CTR_ERR]));
+        }
+        // the partitioned relation moves from the context into the server
+        std::swap(R->key, c->r_key);
+        std::swap(R->val, c->r_val);
+        std::swap(R->start, c->r_start);
+        if (R->plan.bf) std::swap(R->filter, c->filter);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->rel_bytes += R->key.bytes + R->val.bytes + R->start.bytes + R->filter.bytes;
+        s->rels.push_back(R);
+        *rel_id = (int32_t)s->rels.size() - 1;
+        return BJ_OK;
+    });
 }
 
 int bj_server_request_bytes(const bj_server* s, int32_t rel, int64_t n_s, uint64_t* bytes)
@@ -461,58 +465,62 @@ int bj_server_request_bytes(const bj_server* s, int32_t rel, int64_t n_s, uint64
 int bj_server_submit(bj_server* s, int32_t rel, const uint32_t* s_key, const uint32_t* s_val, int64_t n_s,
                      uint64_t* id)
 {
-    if (!s || !id || n_s < 0 || (n_s > 0 && (!s_key || !s_val))) return fail(BJ_ERR_INVALID, "bad argument");
-    const Relation* R = find_rel(s, rel);
-    if (!R) return fail(BJ_ERR_INVALID, "no such relation");
-    Request* q = new Request();
-    int rc = make_request_plan(s, *R, (uint64_t)n_s, q->pl);
-    if (rc == BJ_OK) rc = request_items(q->pl, q->items, q->ws_bytes);
-    if (rc) {
-        delete q;
-        return rc;
-    }
-    q->cap = std::max<uint64_t>(1, (uint64_t)((double)n_s * std::max(1.0, s->opts.answer_scale)));
-    q->out_bytes = 3 * align_up(q->cap * 4);
-    if (align_up(q->ws_bytes) + align_up(q->out_bytes) > s->alloc.size()) {
-        delete q;
-        return fail(BJ_ERR_SIZE, "request footprint exceeds the server arena");
-    }
-    q->rel = rel;
-    q->relp = R;
-    q->sk = s_key;
-    q->sv = s_val;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        q->id = s->next_id++;
-        q->res.id = q->id;
-        q->res.relation = rel;
-        q->res.s_size = (uint64_t)n_s;
-        q->res.capacity = q->cap;
-        q->res.t_submit = s->now();
-        s->reqs[q->id] = q;
-        s->queue.push_back(q);
-        s->submitted++;
-        *id = q->id;
-    }
-    s->cv_work.notify_all();
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!s || !id || n_s < 0 || (n_s > 0 && (!s_key || !s_val))) return fail(BJ_ERR_INVALID, "bad argument");
+        const Relation* R = find_rel(s, rel);
+        if (!R) return fail(BJ_ERR_INVALID, "no such relation");
+        Request* q = new Request();
+        int rc = make_request_plan(s, *R, (uint64_t)n_s, q->pl);
+        if (rc == BJ_OK) rc = request_items(q->pl, q->items, q->ws_bytes);
+        if (rc) {
+            delete q;
+            return rc;
+        }
+        q->cap = std::max<uint64_t>(1, (uint64_t)((double)n_s * std::max(1.0, s->opts.answer_scale)));
+        q->out_bytes = 3 * align_up(q->cap * 4);
+        if (align_up(q->ws_bytes) + align_up(q->out_bytes) > s->alloc.size()) {
+            delete q;
+            return fail(BJ_ERR_SIZE, "request footprint exceeds the server arena");
+        }
+        q->rel = rel;
+        q->relp = R;
+        q->sk = s_key;
+        q->sv = s_val;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            q->id = s->next_id++;
+            q->res.id = q->id;
+            q->res.relation = rel;
+            q->res.s_size = (uint64_t)n_s;
+            q->res.capacity = q->cap;
+            q->res.t_submit = s->now();
+            s->reqs[q->id] = q;
+            s->queue.push_back(q);
+            s->submitted++;
+            *id = q->id;
+        }
+        s->cv_work.notify_all();
+        return BJ_OK;
+    });
 }
 
 int bj_server_poll(bj_server* s, bj_request_result* out, int cap, int* n)
 {
-    if (!s || !n || (cap > 0 && !out)) return fail(BJ_ERR_INVALID, "bad argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    int k = 0;
-    for (auto& kv : s->reqs) {
-        if (k >= cap) break;
-        Request* q = kv.second;
-        if (q->done && !q->reported) {
-            q->reported = true;
-            out[k++] = q->res;
+    return guarded([&]() -> int {
+        if (!s || !n || (cap > 0 && !out)) return fail(BJ_ERR_INVALID, "bad argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        int k = 0;
+        for (auto& kv : s->reqs) {
+            if (k >= cap) break;
+            Request* q = kv.second;
+            if (q->done && !q->reported) {
+                q->reported = true;
+                out[k++] = q->res;
+            }
         }
-    }
-    *n = k;
-    return BJ_OK;
+        *n = k;
+        return BJ_OK;
+    });
 }
 
 int bj_server_wait(bj_server* s, uint64_t id, bj_request_result* out)
@@ -539,21 +547,23 @@ int bj_server_wait(bj_server* s, uint64_t id, bj_request_result* out)
 int bj_server_copy_rows(bj_server* s, const bj_request_result* r, uint32_t* key, uint32_t* r_val, uint32_t* s_val,
                         uint64_t n)
 {
-    if (!s || !r) return fail(BJ_ERR_INVALID, "bad argument");
-    if (n > r->rows_written) return fail(BJ_ERR_INVALID, "n exceeds rows_written");
-    if (n == 0) return BJ_OK;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        auto it = s->reqs.find(r->id);
-        if (it == s->reqs.end() || !it->second->done) return fail(BJ_ERR_INVALID, "no such completed request");
-    }
-    hipStream_t st = s->main->stream;
-    BJ_HIP(hipSetDevice(s->device));
-    if (key) BJ_HIP(hipMemcpyAsync(key, r->key, n * 4, hipMemcpyDeviceToHost, st));
-    if (r_val) BJ_HIP(hipMemcpyAsync(r_val, r->r_val, n * 4, hipMemcpyDeviceToHost, st));
-    if (s_val) BJ_HIP(hipMemcpyAsync(s_val, r->s_val, n * 4, hipMemcpyDeviceToHost, st));
-    BJ_HIP(hipStreamSynchronize(st));
-    return BJ_OK;
+    return guarded([&]() -> int {
+        if (!s || !r) return fail(BJ_ERR_INVALID, "bad argument");
+        if (n > r->rows_written) return fail(BJ_ERR_INVALID, "n exceeds rows_written");
+        if (n == 0) return BJ_OK;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            auto it = s->reqs.find(r->id);
+            if (it == s->reqs.end() || !it->second->done) return fail(BJ_ERR_INVALID, "no such completed request");
+        }
+        hipStream_t st = s->main->stream;
+        BJ_HIP(hipSetDevice(s->device));
+        if (key) BJ_HIP(hipMemcpyAsync(key, r->key, n * 4, hipMemcpyDeviceToHost, st));
+        if (r_val) BJ_HIP(hipMemcpyAsync(r_val, r->r_val, n * 4, hipMemcpyDeviceToHost, st));
+        if (s_val) BJ_HIP(hipMemcpyAsync(s_val, r->s_val, n * 4, hipMemcpyDeviceToHost, st));
+        BJ_HIP(hipStreamSynchronize(st));
+        return BJ_OK;
+    });
 }
 
 int bj_server_release(bj_server* s, uint64_t id)
