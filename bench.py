@@ -377,6 +377,29 @@ def run_serve(args, opts: dict) -> dict:
         nxt = arrivals[k][0] - (time.perf_counter() - start) if k < len(arrivals) else 0.0005
         time.sleep(max(0.0, min(nxt, 0.0005)))
     wall = time.perf_counter() - start
+
+    # capacity: the same request mix submitted at once (about 10 s of the single-join
+    # path's work), the server admitting and overlapping as slots and arena allow
+    sat_n = max(len(pool), int(10.0 / (mean_plain * 1e-3)))
+    sat_jobs = [arrivals[i % len(arrivals)][1] if arrivals else i % len(pool) for i in range(sat_n)]
+    print(f"[serve] saturation: {sat_n} requests at once", file=sys.stderr, flush=True)
+    sat_ids = {}
+    t_sat = time.perf_counter()
+    for j in sat_jobs:
+        ri, sk, sv, ns, m = pool[j]
+        sat_ids[srv.submit(ri, sk, sv, ns)] = (j, m)
+    sat_tuples, sat_done = 0, 0
+    while sat_done < sat_n:
+        for r in srv.poll(512):
+            j, m = sat_ids.pop(r.id)
+            if r.status != 0 or r.match_count != m:
+                bad += 1
+            sat_tuples += pool[j][3] + rel_sizes[pool[j][0]]
+            sat_done += 1
+            srv.release(r.id)
+        time.sleep(0.0002)
+    sat_wall = time.perf_counter() - t_sat
+    sat_plain_s = sum(plain_ms[j] for j in sat_jobs) * 1e-3
     st = srv.stats()
     srv.close()
     if bad:
@@ -413,6 +436,9 @@ def run_serve(args, opts: dict) -> dict:
                    "max_in_flight": int(st.max_in_flight), "arena_peak_bytes": int(st.arena_peak_bytes),
                    "relation_bytes": int(st.relation_bytes), "register_s": round(reg_s, 3),
                    "alone_ms_mean": round(float(np.mean(alone_ms)), 3)},
+        "saturated": {"requests": sat_n, "wall_s": round(sat_wall, 3),
+                      "request_input_tuples_per_s": sat_tuples / sat_wall,
+                      "vs_plain_same_requests": round(sat_plain_s / sat_wall, 3)},
         "plain_per_request": {"service_ms_mean": round(mean_plain, 3),
                               "capacity_tuples_per_s": tot_plain_tuples / (sum(plain_ms[j] for _, j in arrivals) * 1e-3),
                               "p50_ms_same_arrivals": round(float(np.percentile(plain_lat, 50)), 3),
