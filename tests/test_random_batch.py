@@ -99,3 +99,22 @@ def test_random_batch(gpu_ctx, seed):
                 assert o.match_count == n and not o.truncated, (rep, i)
                 k, a, c = b.rows_to_host(o)
                 assert np.array_equal(O.sort_rows(k, a, c), rows), (rep, i)
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_empty_and_degenerate_batches(gpu_ctx, use_graph):
+    """No joins at all, and joins whose sides are all empty, run and report nothing."""
+    with bj.Batch(gpu_ctx, [], use_graph=bool(use_graph)) as b:
+        for _ in range(2):
+            outs, _ = b.run()
+            assert outs == []
+    z = _dev(np.zeros(0, np.uint32))
+    one = _dev(np.array([5], np.uint32))
+    specs = [bj.join_spec(z, z, z, z, r_size=0, s_size=0),
+             bj.join_spec(one, one, z, z, r_size=1, s_size=0, bf_bits_per_key=10.0),
+             bj.join_spec(z, z, one, one, r_size=0, s_size=1, radix_bits=9)]
+    with bj.Batch(gpu_ctx, specs, use_graph=bool(use_graph)) as b:
+        for _ in range(2):
+            outs, _ = b.run()
+            assert [o.match_count for o in outs] == [0, 0, 0]
+            assert all(o.rows_written == 0 and not o.truncated for o in outs)
