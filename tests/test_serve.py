@@ -147,3 +147,25 @@ def test_serve_matches_single_join_and_limits(gpu_ctx):
         srv.drain()
         st = srv.stats()
         assert st.completed == st.submitted == st.released == 13
+
+
+@pytest.mark.gpu
+def test_serve_degenerate_requests(gpu_ctx):
+    """Empty S, a one-tuple S, an empty resident relation: each result equals the oracle's."""
+    rk, rv = _gen(gpu_ctx, 31, 1000, 0, 0)
+    sk, sv, m = _gen_s(gpu_ctx, 31, 32, 1000, 5000, 0.5)
+    hr = [_host(rk, 1000), _host(rv, 1000)]
+    hs = [_host(sk, 5000), _host(sv, 5000)]
+    with bj.Server(0, slots=2) as srv:
+        rel = srv.add_relation(rk, rv, 1000)
+        empty_rel = srv.add_relation(rk, rv, 0)
+        cases = [(rel, 0), (rel, 1), (rel, 5000), (empty_rel, 5000), (empty_rel, 0)]
+        ids = [srv.submit(r, sk, sv, n) for r, n in cases]
+        for (r, n), rid in zip(cases, ids):
+            res = srv.wait(rid)
+            nr = 1000 if r == rel else 0
+            cnt, k, a, b = O.cpu_join(hr[0][:nr], hr[1][:nr], hs[0][:n], hs[1][:n])
+            assert res.status == 0 and res.match_count == cnt and not res.truncated, (r, n)
+            assert np.array_equal(O.sort_rows(*srv.rows_to_host(res)), O.sort_rows(k, a, b)), (r, n)
+            srv.release(rid)
+    assert m == O.match_count(5000, 0.5)
