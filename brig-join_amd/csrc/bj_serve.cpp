@@ -22,6 +22,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -393,11 +394,10 @@ int bj_server_add_relation(bj_server* s, const uint32_t* r_key, const uint32_t* 
         p.bf_bits_per_key = s->opts.bf_bits_per_key;
         p.num_hashes = s->opts.num_hashes;
         p.inputs_on_device = 1;
-        Relation* R = new Relation();
+        std::unique_ptr<Relation> R(new Relation());
         R->nr = (uint64_t)n_r;
         int rc = make_plan(p, R->plan);
         if (rc) {
-            delete R;
             return rc;
         }
         Plan pl = R->plan;
@@ -406,31 +406,25 @@ int bj_server_add_relation(bj_server* s, const uint32_t* r_key, const uint32_t* 
         bj_ctx* c = s->main;
         hipError_t e = hipSetDevice(s->device);
         if (e != hipSuccess) {
-            delete R;
             return fail(BJ_ERR_HIP, "hipSetDevice failed");
         }
         if ((rc = ensure_workspace(c, pl))) {
-            delete R;
             return rc;
         }
         tl_reset(c, false);
         e = launch_zero_u64(P<uint64_t>(c->counters), CTR_N, c->stream);
         if (e != hipSuccess) {
-            delete R;
             return fail(BJ_ERR_HIP, std::string("relation counters: ") + hipGetErrorString(e));
         }
         if ((rc = enqueue_r(c, pl, r_key, r_val))) {
-            delete R;
             return rc;
         }
         e = hipMemcpyAsync(c->h_counters, c->counters.p, CTR_N * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) {
-            delete R;
             return fail(BJ_ERR_HIP, std::string("relation partition: ") + hipGetErrorString(e));
         }
         if (c->h_counters[CTR_ERR]) {
-            delete R;
             return fail(BJ_ERR_INTERNAL, "device check flags " + std::to_string(c->h_counters[CTR_ERR]));
         }
         // the partitioned relation moves from the context into the server
@@ -440,7 +434,8 @@ int bj_server_add_relation(bj_server* s, const uint32_t* r_key, const uint32_t* 
         if (R->plan.bf) std::swap(R->filter, c->filter);
         std::lock_guard<std::mutex> lk(s->mu);
         s->rel_bytes += R->key.bytes + R->val.bytes + R->start.bytes + R->filter.bytes;
-        s->rels.push_back(R);
+        s->rels.push_back(R.get());  // on a throw R still owns the relation and frees it
+        R.release();                 // owned by the server from here on
         *rel_id = (int32_t)s->rels.size() - 1;
         return BJ_OK;
     });
@@ -469,19 +464,14 @@ int bj_server_submit(bj_server* s, int32_t rel, const uint32_t* s_key, const uin
         if (!s || !id || n_s < 0 || (n_s > 0 && (!s_key || !s_val))) return fail(BJ_ERR_INVALID, "bad argument");
         const Relation* R = find_rel(s, rel);
         if (!R) return fail(BJ_ERR_INVALID, "no such relation");
-        Request* q = new Request();
+        std::unique_ptr<Request> q(new Request());
         int rc = make_request_plan(s, *R, (uint64_t)n_s, q->pl);
         if (rc == BJ_OK) rc = request_items(q->pl, q->items, q->ws_bytes);
-        if (rc) {
-            delete q;
-            return rc;
-        }
+        if (rc) return rc;
         q->cap = std::max<uint64_t>(1, (uint64_t)((double)n_s * std::max(1.0, s->opts.answer_scale)));
         q->out_bytes = 3 * align_up(q->cap * 4);
-        if (align_up(q->ws_bytes) + align_up(q->out_bytes) > s->alloc.size()) {
-            delete q;
+        if (align_up(q->ws_bytes) + align_up(q->out_bytes) > s->alloc.size())
             return fail(BJ_ERR_SIZE, "request footprint exceeds the server arena");
-        }
         q->rel = rel;
         q->relp = R;
         q->sk = s_key;
@@ -494,10 +484,15 @@ int bj_server_submit(bj_server* s, int32_t rel, const uint32_t* s_key, const uin
             q->res.s_size = (uint64_t)n_s;
             q->res.capacity = q->cap;
             q->res.t_submit = s->now();
-            s->reqs[q->id] = q;
-            s->queue.push_back(q);
+            auto it = s->reqs.emplace(q->id, q.get()).first;
+            try {
+                s->queue.push_back(q.get());
+            } catch (...) {
+                s->reqs.erase(it);
+                throw;
+            }
+            *id = q.release()->id;  // owned by the server from here on
             s->submitted++;
-            *id = q->id;
         }
         s->cv_work.notify_all();
         return BJ_OK;
