@@ -166,6 +166,9 @@ int make_plan(const bj_params& p, Plan& pl)
         return fail(BJ_ERR_INVALID, "unsupported use_bf=" + std::to_string(p.use_bf));
     pl.bf = p.use_bf != 0;
     pl.probe_breakdown = p.probe_breakdown != 0;
+    if (pl.bf && !std::isfinite(p.bf_bits_per_key))
+        return fail(BJ_ERR_INVALID, "bf_bits_per_key must be a finite number");
+    if (p.s_chunks < 0) return fail(BJ_ERR_INVALID, "s_chunks must be >= 0");
 
     int b1 = -1, b2 = -1;
     if (p.partition_size >= 0) {
@@ -290,8 +293,7 @@ int make_plan(const bj_params& p, Plan& pl)
     pl.G_s = choose_G(pl.ns);
     pl.solo_r = pl.nr >= SOLO_TUPLES;
     pl.solo_s = pl.ns >= SOLO_TUPLES;
-    const double scale = std::max(1.0, p.answer_scale);
-    pl.cap = std::max<uint64_t>(1, (uint64_t)((double)std::max(pl.nr, pl.ns) * scale));
+    if (const int rc = row_capacity(std::max(pl.nr, pl.ns), p.answer_scale, pl.cap)) return rc;
     // sum over sub-buckets of ceil(nb / tile) <= (nr + ns) / tile + 2^bits
     pl.item_cap = (uint64_t)pl.max_slices * ((pl.nr + pl.ns) / pl.tile + (1ull << total)) + 16;
     return BJ_OK;
@@ -623,33 +625,40 @@ int bj_create(bj_ctx** out, int device)
         if (device < 0 || device >= ndev) return fail(BJ_ERR_NODEVICE, "device index out of range");
         BJ_HIP(hipSetDevice(device));
         bj_ctx* c = new bj_ctx();
-        c->device = device;
-        int cu = 0;
-        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
-            c->num_cu = cu;
-        int max_lds = 64 * 1024;
-        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
-            max_lds = 64 * 1024;
-        c->max_lds = max_lds;
-        hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = kernels_init(max_lds);
-        // the scratch array of bj_join_noalloc on this device (its failure only
-        // disables that entry point, which then reports it)
-        if (e == hipSuccess) (void)noalloc_init(device);
-        if (e == hipSuccess) e = batch_kernels_init(max_lds);
-        c->ev.assign(EV_N, nullptr);
-        for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_counters, (CTR_N + 1) * sizeof(uint64_t), hipHostMallocDefault);
-        if (e != hipSuccess) {
+        // a throw past this point (the event vector) destroys the half-built context
+        try {
+            c->device = device;
+            int cu = 0;
+            if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
+                c->num_cu = cu;
+            int max_lds = 64 * 1024;
+            if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+                max_lds = 64 * 1024;
+            c->max_lds = max_lds;
+            hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+            if (e == hipSuccess) e = kernels_init(max_lds);
+            // the scratch array of bj_join_noalloc on this device (its failure only
+            // disables that entry point, which then reports it)
+            if (e == hipSuccess) (void)noalloc_init(device);
+            if (e == hipSuccess) e = batch_kernels_init(max_lds);
+            c->ev.assign(EV_N, nullptr);
+            for (int i = 0; i < EV_N && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+            if (e == hipSuccess)
+                e = hipHostMalloc((void**)&c->h_counters, (CTR_N + 1) * sizeof(uint64_t), hipHostMallocDefault);
+            if (e != hipSuccess) {
+                bj_destroy(c);
+                return fail(BJ_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
+            }
+            c->stream = c->own_stream;
+            int rc = ensure(c->counters, CTR_N * sizeof(uint64_t));
+            if (rc == BJ_OK) rc = ensure(c->dtmp, 64);
+            if (rc != BJ_OK) {
+                bj_destroy(c);
+                return rc;
+            }
+        } catch (...) {
             bj_destroy(c);
-            return fail(BJ_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
-        }
-        c->stream = c->own_stream;
-        int rc = ensure(c->counters, CTR_N * sizeof(uint64_t));
-        if (rc == BJ_OK) rc = ensure(c->dtmp, 64);
-        if (rc != BJ_OK) {
-            bj_destroy(c);
-            return rc;
+            throw;
         }
         *out = c;
         return BJ_OK;

@@ -121,6 +121,7 @@ int resolve(const bj_ctx* c, const bj_join_spec& s, JoinCfg& o)
     o.b1 = o.bits <= B_ONE_PASS_BITS ? o.bits : (o.bits + 1) / 2;
 
     double bpk = s.bf_bits_per_key;
+    if (!std::isfinite(bpk)) return fail(BJ_ERR_INVALID, "bf_bits_per_key must be a finite number");
     if (bpk < 0.0) {
         // the tuner's filter classes: filters that stay in the L2 (lg 0) and
         // filters probed after a first pass of S (lg 1), told apart by the size
@@ -153,9 +154,7 @@ int resolve(const bj_ctx* c, const bj_join_spec& s, JoinCfg& o)
     } else {
         o.k = 0;
     }
-    const double scale = std::max(1.0, s.answer_scale);
-    o.cap = std::max<uint64_t>(1, (uint64_t)((double)std::max(nr, ns) * scale));
-    return BJ_OK;
+    return row_capacity(std::max(nr, ns), s.answer_scale, o.cap);
 }
 
 // Histogram / scatter chunk of a relation: at least 32K tuples, at most 512
@@ -562,53 +561,66 @@ int bj_batch_create(bj_ctx* c, int n, const bj_join_spec* specs, int use_graph, 
                 return fail(BJ_ERR_INVALID, "join " + std::to_string(i) + ": null relation pointer");
         }
         bj_batch* b = new bj_batch();
-        b->c = c;
-        b->n = n;
-        int rc = make_layout(c, n, specs, b->L);
-        if (rc) {
-            delete b;
-            return rc;
-        }
-        hipError_t e = hipSetDevice(c->device);
-        if (e == hipSuccess) e = hipEventCreate(&b->e0);
-        if (e == hipSuccess) e = hipEventCreate(&b->e1);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_ctr, ctr_words(n) * 8, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            bj_batch_destroy(b);
-            return fail(BJ_ERR_HIP, std::string("batch setup: ") + hipGetErrorString(e));
-        }
-        if ((rc = batch_workspace(b, b->L, n, nullptr))) {
-            bj_batch_destroy(b);
-            return rc;
-        }
-        hipStream_t st = c->stream;
-        e = hipMemcpyAsync(b->dJ.p, b->L.J.data(), sizeof(BJoin) * n, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->dpx.p, b->L.px.data(), b->L.px.size() * 4, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
-            bj_batch_destroy(b);
-            return fail(BJ_ERR_HIP, std::string("batch descriptors: ") + hipGetErrorString(e));
-        }
-        if (use_graph) {
-            e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        // a throw past this point (the layout's vectors, the enqueue under
+        // capture) ends an open capture and destroys the half-built batch
+        try {
+            b->c = c;
+            b->n = n;
+            int rc = make_layout(c, n, specs, b->L);
+            if (rc) {
+                delete b;
+                return rc;
+            }
+            hipError_t e = hipSetDevice(c->device);
+            if (e == hipSuccess) e = hipEventCreate(&b->e0);
+            if (e == hipSuccess) e = hipEventCreate(&b->e1);
+            if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_ctr, ctr_words(n) * 8, hipHostMallocDefault);
             if (e != hipSuccess) {
                 bj_batch_destroy(b);
-                return fail(BJ_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+                return fail(BJ_ERR_HIP, std::string("batch setup: ") + hipGetErrorString(e));
             }
-            rc = enqueue(b, st);
-            hipGraph_t g = nullptr;
-            e = hipStreamEndCapture(st, &g);
-            b->graph = g;
-            if (rc) {
+            if ((rc = batch_workspace(b, b->L, n, nullptr))) {
                 bj_batch_destroy(b);
                 return rc;
             }
-            if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+            hipStream_t st = c->stream;
+            e = hipMemcpyAsync(b->dJ.p, b->L.J.data(), sizeof(BJoin) * n, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b->dpx.p, b->L.px.data(), b->L.px.size() * 4, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) {
                 bj_batch_destroy(b);
-                return fail(BJ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+                return fail(BJ_ERR_HIP, std::string("batch descriptors: ") + hipGetErrorString(e));
             }
+            if (use_graph) {
+                e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) {
+                    bj_batch_destroy(b);
+                    return fail(BJ_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+                }
+                rc = enqueue(b, st);
+                hipGraph_t g = nullptr;
+                e = hipStreamEndCapture(st, &g);
+                b->graph = g;
+                if (rc) {
+                    bj_batch_destroy(b);
+                    return rc;
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    bj_batch_destroy(b);
+                    return fail(BJ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+                }
+            }
+        } catch (...) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(c->stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(c->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+            }
+            bj_batch_destroy(b);
+            throw;
         }
         *out = b;
         return BJ_OK;

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
 #include <exception>
 #include <functional>
 #include <new>
@@ -34,6 +36,20 @@ int guarded(F&& body) noexcept
     } catch (...) {
         return fail(BJ_ERR_INTERNAL, "host exception");
     }
+}
+
+// Row capacity max(1, n * max(1, answer_scale)) of a join with n = max(R, S):
+// a scale that is not a finite number is refused, and so is a capacity above
+// 2^40 rows (12 TiB of rows), which no device holds (and whose conversion to
+// an integer would overflow).
+constexpr uint64_t MAX_ROWS = 1ull << 40;
+inline int row_capacity(uint64_t n, double answer_scale, uint64_t& cap)
+{
+    if (!std::isfinite(answer_scale)) return fail(BJ_ERR_INVALID, "answer_scale must be a finite number");
+    const double c = (double)n * std::max(1.0, answer_scale);
+    if (c > (double)MAX_ROWS) return fail(BJ_ERR_SIZE, "row capacity (relation size x answer_scale) above 2^40 rows");
+    cap = std::max<uint64_t>(1, (uint64_t)c);
+    return BJ_OK;
 }
 
 #define BJ_HIP(expr)                                                                         \

@@ -270,6 +270,35 @@ void scheduler(bj_server* s)
     }
 }
 
+// The arena, the slots and the scheduler of a new server (its options checked);
+// on an error the caller destroys the half-built server.
+int build_server(bj_server* s, int device, const bj_server_opts& opts)
+{
+    int rc;
+    s->t_origin = std::chrono::steady_clock::now();
+    uint64_t arena = opts.arena_bytes;
+    if (arena == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(BJ_ERR_HIP, "hipMemGetInfo failed");
+        arena = fr / 2;
+    }
+    arena = align_up(arena);
+    if ((rc = ensure(s->arena, arena))) return rc;
+    s->alloc.reset(arena);
+    s->slots.resize(opts.slots);
+    for (Slot& sl : s->slots) {
+        if ((rc = bj_create(&sl.c, device))) return rc;
+        release(sl.c->counters);  // a request brings its counters from the arena
+        hipError_t e = hipEventCreate(&sl.t0);
+        if (e == hipSuccess) e = hipEventCreate(&sl.t1);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_ctr, CTR_N * sizeof(uint64_t), hipHostMallocDefault);
+        if (e != hipSuccess) return fail(BJ_ERR_HIP, std::string("server slot: ") + hipGetErrorString(e));
+    }
+    s->sched = std::thread(scheduler, s);
+    return BJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -334,47 +363,34 @@ int bj_server_create(int device, const bj_server_opts* o, bj_server** out)
         if (o) opts = *o;
         if (opts.slots < 1 || opts.slots > SRV_MAX_SLOTS) return fail(BJ_ERR_INVALID, "slots must be in [1, 8]");
         if (opts.radix_bits == 0 || opts.radix_bits > 15) return fail(BJ_ERR_INVALID, "radix_bits must be -1 or 1..15");
-        if (opts.use_bf && !(opts.bf_bits_per_key > 0.0)) return fail(BJ_ERR_INVALID, "bf_bits_per_key must be > 0");
+        if (opts.use_bf && !(opts.bf_bits_per_key > 0.0 && std::isfinite(opts.bf_bits_per_key)))
+            return fail(BJ_ERR_INVALID, "bf_bits_per_key must be a finite number > 0");
+        if (!std::isfinite(opts.answer_scale)) return fail(BJ_ERR_INVALID, "answer_scale must be a finite number");
         bj_ctx* main = nullptr;
         int rc = bj_create(&main, device);
         if (rc) return rc;
-        bj_server* s = new bj_server();
+        bj_server* s = nullptr;
+        try {
+            s = new bj_server();
+        } catch (...) {
+            bj_destroy(main);
+            throw;
+        }
         s->device = device;
         s->opts = opts;
         s->main = main;
-        s->t_origin = std::chrono::steady_clock::now();
-        uint64_t arena = opts.arena_bytes;
-        if (arena == 0) {
-            size_t fr = 0, tot = 0;
-            if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
-                bj_server_destroy(s);
-                return fail(BJ_ERR_HIP, "hipMemGetInfo failed");
-            }
-            arena = fr / 2;
+        // from here on the server owns everything: a throw (slot vector,
+        // scheduler thread) tears down what was built so far
+        try {
+            rc = build_server(s, device, opts);
+        } catch (...) {
+            bj_server_destroy(s);
+            throw;
         }
-        arena = align_up(arena);
-        if ((rc = ensure(s->arena, arena))) {
+        if (rc) {
             bj_server_destroy(s);
             return rc;
         }
-        s->alloc.reset(arena);
-        s->slots.resize(opts.slots);
-        for (Slot& sl : s->slots) {
-            if ((rc = bj_create(&sl.c, device))) {
-                bj_server_destroy(s);
-                return rc;
-            }
-            release(sl.c->counters);  // a request brings its counters from the arena
-            hipError_t e = hipEventCreate(&sl.t0);
-            if (e == hipSuccess) e = hipEventCreate(&sl.t1);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-            if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_ctr, CTR_N * sizeof(uint64_t), hipHostMallocDefault);
-            if (e != hipSuccess) {
-                bj_server_destroy(s);
-                return fail(BJ_ERR_HIP, std::string("server slot: ") + hipGetErrorString(e));
-            }
-        }
-        s->sched = std::thread(scheduler, s);
         *out = s;
         return BJ_OK;
     });
@@ -452,7 +468,8 @@ int bj_server_request_bytes(const bj_server* s, int32_t rel, int64_t n_s, uint64
     std::vector<Item> items;
     uint64_t ws = 0;
     if ((rc = request_items(pl, items, ws))) return rc;
-    const uint64_t cap = std::max<uint64_t>(1, (uint64_t)((double)n_s * std::max(1.0, s->opts.answer_scale)));
+    uint64_t cap = 0;
+    if ((rc = row_capacity((uint64_t)n_s, s->opts.answer_scale, cap))) return rc;
     *bytes = ws + 3 * align_up(cap * 4);
     return BJ_OK;
 }
@@ -468,7 +485,7 @@ int bj_server_submit(bj_server* s, int32_t rel, const uint32_t* s_key, const uin
         int rc = make_request_plan(s, *R, (uint64_t)n_s, q->pl);
         if (rc == BJ_OK) rc = request_items(q->pl, q->items, q->ws_bytes);
         if (rc) return rc;
-        q->cap = std::max<uint64_t>(1, (uint64_t)((double)n_s * std::max(1.0, s->opts.answer_scale)));
+        if ((rc = row_capacity((uint64_t)n_s, s->opts.answer_scale, q->cap))) return rc;
         q->out_bytes = 3 * align_up(q->cap * 4);
         if (align_up(q->ws_bytes) + align_up(q->out_bytes) > s->alloc.size())
             return fail(BJ_ERR_SIZE, "request footprint exceeds the server arena");

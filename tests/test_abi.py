@@ -101,6 +101,13 @@ def test_plan_reference_bloom_formula():
     (dict(r_size=2**31), -5),
     (dict(r_size=-1), -1),
     (dict(partition_size=512, second_partition_size=32, radix_bits=12), -1),
+    # not finite numbers, and a row capacity whose conversion would overflow
+    (dict(use_bf=1, bf_bits_per_key=float("nan")), -1),
+    (dict(use_bf=1, bf_bits_per_key=float("inf")), -1),
+    (dict(answer_scale=float("nan")), -1),
+    (dict(answer_scale=float("inf")), -1),
+    (dict(answer_scale=1e30), -5),
+    (dict(s_chunks=-2), -1),
 ])
 def test_plan_rejects_invalid(kw, code):
     base = dict(r_size=1000, s_size=1000)

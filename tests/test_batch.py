@@ -70,6 +70,8 @@ def test_plan_filter_stage_by_size():
 
 @pytest.mark.parametrize("kw,code", [
     (dict(tile=1000), -1), (dict(radix_bits=16), -5), (dict(num_hashes=9, bf_bits_per_key=8.0), -1),
+    (dict(bf_bits_per_key=float("nan")), -1), (dict(bf_bits_per_key=float("inf")), -1),
+    (dict(answer_scale=float("inf")), -1), (dict(answer_scale=1e30), -5),
 ])
 def test_plan_rejects_bad_specs(kw, code):
     with pytest.raises(bj.BjError) as e:

@@ -24,6 +24,11 @@ def test_server_opts_default_and_nodevice():
         with pytest.raises(bj.BjError) as e:
             bj.Server(0)
         assert e.value.code == -6
+    # options are checked before the device is opened
+    for bad in (dict(answer_scale=float("nan")), dict(bf_bits_per_key=float("inf")), dict(slots=0)):
+        with pytest.raises(bj.BjError) as e:
+            bj.Server(0, **bad)
+        assert e.value.code == -1, bad
 
 
 def _gen(ctx, seed, nr, ns, ratio):
