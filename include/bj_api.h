@@ -107,12 +107,14 @@ typedef struct bj_params {
                                        R is already complete, so every reference mode
                                        collapses to "on")                              */
     double  bf_bits_per_key;        /* > 0: filter bits per R key; <= 0: derive from
-                                       true_false_ratio with the reference formula     */
+                                       true_false_ratio with the reference formula
+                                       (a finite number when the filter is on)         */
     double  true_false_ratio;       /* argv[11] target false-positive rate p          */
     int64_t num_bits;               /* --num_bits override (filter bits), -1 = none   */
     int32_t num_hashes;             /* --num_hashes override (k, 1..8), -1 = none     */
     double  answer_scale;           /* --answer_scale: row capacity =
-                                       max(R,S) * max(1, answer_scale)                */
+                                       max(R,S) * max(1, answer_scale); a finite
+                                       number, capacities above 2^40 rows refused     */
     int32_t tile;                   /* LDS build tile in tuples (4096 or 8192); 0=auto */
     int32_t inputs_on_device;       /* 1: r_key.. are device pointers; 0: host         */
     int32_t collect_phase_times;    /* 1: fill bj_result.phase_ms                      */
@@ -373,7 +375,8 @@ typedef struct bj_join_spec {
     int32_t tile;                    /* 4096 or 8192; 0 = autotuner / 8192       */
     double  bf_bits_per_key;         /* > 0: filter on; 0: off; < 0: autotuner   */
     int32_t num_hashes;              /* 1..8; -1 = min(round(bits_per_key ln 2), 4) */
-    double  answer_scale;            /* row capacity = max(R,S) * max(1, scale)  */
+    double  answer_scale;            /* row capacity = max(R,S) * max(1, scale),
+                                        finite, at most 2^40 rows               */
     double  match_hint;              /* expected fraction of S with a match, used
                                         only to pick the autotuner's table entry;
                                         < 0 = unknown (taken as 0.1)             */
@@ -454,7 +457,8 @@ typedef struct bj_server_opts {
     double   bf_bits_per_key;  /* default 10                                        */
     int32_t  num_hashes;       /* default 3                                         */
     int32_t  radix_bits;       /* per relation, 1..15; -1 = from |R| (default)      */
-    double   answer_scale;     /* row capacity of a request = |S| * max(1, scale)   */
+    double   answer_scale;     /* row capacity of a request = |S| * max(1, scale),
+                                  finite, at most 2^40 rows                         */
 } bj_server_opts;
 
 typedef struct bj_request_result {
