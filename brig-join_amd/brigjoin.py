@@ -314,10 +314,12 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter shutdown the HIP runtime may already be going away:
         # leave the memory to the end of the process instead of freeing it
-        if sys.is_finalizing():
+        # (the check is bound at definition: module globals such as sys may
+        # already be cleared when this runs)
+        if _finalizing():
             return
         try:
             self.close()
@@ -448,10 +450,12 @@ class Batch:
     def __exit__(self, *a):
         self.close()
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter shutdown the HIP runtime may already be going away:
         # leave the memory to the end of the process instead of freeing it
-        if sys.is_finalizing():
+        # (the check is bound at definition: module globals such as sys may
+        # already be cleared when this runs)
+        if _finalizing():
             return
         try:
             self.close()
@@ -558,10 +562,12 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter shutdown the HIP runtime may already be going away:
         # leave the memory to the end of the process instead of freeing it
-        if sys.is_finalizing():
+        # (the check is bound at definition: module globals such as sys may
+        # already be cleared when this runs)
+        if _finalizing():
             return
         try:
             self.close()
