@@ -8,9 +8,12 @@ to filter S.  Inputs are generated on the device and are resident in HBM before
 the timed region.  A step is one full join of the two relations.
 
 value = (N_R + N_S) / (wall seconds per step), timed over K steps between
-torch.cuda.synchronize() calls.  roofline = the dominant kernel's algorithmic
-bytes / its average duration (HIP events on the library's stream, over the timed
-steps) against 8 TB/s.  cpu_baseline = the reference's own CPU join
+torch.cuda.synchronize() calls.  roofline = the dominant kernel (the kernel
+function with the most device time per step, its launches summed, as rocprofv3
+--stats ranks kernels): its algorithmic bytes per launch / its average launch
+duration (HIP events on the library's stream, over the timed steps) against
+8 TB/s; roofline.by_kernel gives the same for every kernel with algorithmic
+bytes.  cpu_baseline = the reference's own CPU join
 (src/sanity_check.cpp compiled into oracle/_ref), or the oracle's restatement
 of it, on the benchmark's own inputs copied to the host, all host threads; the
 oracle's row-emitting restatement is timed beside it.
@@ -67,6 +70,44 @@ def algorithmic_bytes(kernel: str, n_r: int, n_s: int, m: int, m_s: int, bf_byte
         "hist_s": 4.0 * s_first, "scatter1_s": 16.0 * s_first, "scatter2_s": s_second,
         "probe": 8.0 * n_r + 8.0 * s_part + 12.0 * m,
     }.get(kernel, 0.0)
+
+
+# Timing slot -> kernel function (rocprofv3's names without template arguments).
+# The filter slot runs k_filter_l1 after S's first pass (stage 2), else
+# k_bloom_prefilter.
+KERNEL_OF_SLOT = {"hist_r": "k_hist", "hist_s": "k_hist", "scatter1_r": "k_scatter", "scatter2_r": "k_scatter",
+                  "scatter1_s": "k_scatter", "scatter2_s": "k_scatter", "bf_build": "k_bloom_build",
+                  "probe": "k_probe"}
+
+
+def kernel_rooflines(kern_avg: dict, ab, bf_stage: int, workload: str) -> dict:
+    """Per kernel function: its launches per step (slots with algorithmic bytes),
+    device ms per step, algorithmic bytes and HBM traffic per launch, and the
+    achieved rate (algorithmic bytes over device time) against 8 TB/s."""
+    groups = {}
+    for slot, ms in kern_avg.items():
+        a = ab(slot)
+        if a <= 0 or ms <= 0:
+            continue
+        fn = KERNEL_OF_SLOT.get(slot) or ("k_filter_l1" if bf_stage == 2 else "k_bloom_prefilter")
+        g = groups.setdefault(fn, {"slots": [], "ms": 0.0, "alg": 0.0, "traffic": 0, "have_traffic": True})
+        g["slots"].append(slot)
+        g["ms"] += ms
+        g["alg"] += a
+        t = load_traffic(workload, slot)
+        if t is None:
+            g["have_traffic"] = False
+        else:
+            g["traffic"] += t
+    out = {}
+    for fn, g in groups.items():
+        n = len(g["slots"])
+        achieved = g["alg"] / (g["ms"] * 1e-3) / 1e9
+        out[fn] = {"launches": n, "slots": g["slots"], "step_ms": round(g["ms"], 4),
+                   "avg_ms": round(g["ms"] / n, 4), "alg_bytes": g["alg"] / n, "achieved": round(achieved, 1),
+                   "frac": round(achieved / HBM_PEAK_GBS, 4),
+                   "traffic": int(g["traffic"] / n) if g["have_traffic"] else None}
+    return out
 
 
 def total_algorithmic_bytes(n_r, n_s, m, m_s, bf_bytes, bloom) -> float:
@@ -633,10 +674,9 @@ def main() -> int:
     ms_sv = int(res.s_survivors)
     stage = int(res.bf_stage)
     ab = lambda k: algorithmic_bytes(k, n_r, n_s, m, m_s, bf_bytes, bloom, stage)  # noqa: E731
-    dominant = max((k for k in kern_avg if ab(k) > 0), key=lambda k: kern_avg[k])
-    alg = ab(dominant)
-    achieved = alg / (kern_avg[dominant] * 1e-3) / 1e9
-    traffic = load_traffic(args.workload, dominant)
+    by_kernel = kernel_rooflines(kern_avg, ab, stage, args.workload)
+    dominant = max(by_kernel, key=lambda k: by_kernel[k]["step_ms"])
+    dk = by_kernel[dominant]
     per_kernel = {k: {"ms": round(v, 4),
                       "alg_GBps": round(ab(k) / (v * 1e-3) / 1e9, 1) if v > 0 else None}
                   for k, v in kern_avg.items() if v > 0}
@@ -663,9 +703,11 @@ def main() -> int:
                    "bf_stage": int(res.bf_stage), "pass1_bits": int(res.pass1_bits),
                    "s_pass1_bits": int(res.s_pass1_bits),
                    "bf_seg_bits": int(res.bf_seg_bits), "bf_k": int(res.bf_k)},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "alg_bytes": alg, "avg_ms": round(kern_avg[dominant], 4)},
+        "roofline": {"bound": "hbm", "kernel": dominant, "launches": dk["launches"], "achieved": dk["achieved"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["frac"], "traffic": dk["traffic"],
+                     "alg_bytes": dk["alg_bytes"], "avg_ms": dk["avg_ms"],
+                     "step_share": round(dk["step_ms"] / (total_dev / args.steps), 4),
+                     "by_kernel": by_kernel},
         "whole_join": {"alg_bytes": B, "device_ms": round(total_dev / args.steps, 4),
                        "alg_frac_of_8TBps": round(B / (total_dev / args.steps * 1e-3) / 8e12, 4),
                        "matches": m, "s_survivors": ms_sv,
