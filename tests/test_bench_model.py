@@ -4,8 +4,9 @@ The whole-join algorithmic bytes of the three single-join workloads are pinned
 to the closed forms B = 40 (N_R + N_S) + 12 M without the filter and
 B = 44 N_R + F + 8 N_S + 48 M_S + 12 M with it (two partition passes; F the
 filter bytes, M_S the matching S tuples), and to the values the measurement
-plan derived from them by hand (1.299e9, 3.792e10, 1.392e10).  The per-kernel roofline sums a kernel's launches
-per step and names the kernel with the most device time.
+plan derived from them by hand (1.299e9, 3.792e10, 1.392e10).  The per-kernel
+roofline sums a kernel's launches per step and names the kernel with the most
+device time.
 """
 import importlib.util
 import os
